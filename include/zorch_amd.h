@@ -1,0 +1,176 @@
+/* zorch_amd — C ABI of the MI355X dense sumcheck prover (libzorch_amd.so).
+ *
+ * Plain pointers and sizes only. Every field element crosses the boundary in
+ * its storage encoding, little-endian: BabyBear = one u32 Montgomery word
+ * (a * 2^32 mod p), BabyBear^4 = four such words c0..c3 (X^4 - 11),
+ * Goldilocks = one canonical u64. Host buffers are borrowed for the duration
+ * of a call; device pointers (tables_on_device = 1) must stay valid until the
+ * call returns. A zs_ctx owns one HIP stream and is single-threaded; distinct
+ * contexts are independent.
+ *
+ * Return codes: ZS_OK, or an error whose message zs_last_error() returns
+ * (thread-local). The Python layer maps ZS_EINVAL to ValueError,
+ * ZS_ENOMEM / ZS_EBUDGET to MemoryError and ZS_EHIP / ZS_ENODEV to
+ * RuntimeError — the reference's conventions (structural errors raise
+ * ValueError; soundness failures return ok = 0 and never raise).
+ */
+#ifndef ZORCH_AMD_H_
+#define ZORCH_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZS_OK 0
+#define ZS_EINVAL 1
+#define ZS_ENOMEM 2
+#define ZS_EHIP 3
+#define ZS_EBUDGET 4
+#define ZS_ENODEV 5
+
+/* Field ids. */
+#define ZS_F_BABYBEAR 1
+#define ZS_F_BABYBEAR4 2
+#define ZS_F_GOLDILOCKS 3
+
+/* Transcript kinds. */
+#define ZS_TR_SHA256 1
+
+/* Prover modes. */
+#define ZS_MODE_PRODUCT 1 /* StandardRound(ProductSummand(degree)), natural domain */
+#define ZS_MODE_EQ 2      /* EqPolyProver(ProductSummand(degree)), natural domain */
+
+typedef struct zs_ctx zs_ctx;
+typedef struct zs_state zs_state;
+
+/* Streaming SHA-256 state: midstate, pending tail, byte counts (112 bytes). */
+typedef struct zs_sha256_state {
+  uint32_t h[8];
+  uint8_t pending[64];
+  uint64_t pending_len;
+  uint64_t total_len;
+} zs_sha256_state;
+
+/* A field transcript value (128 bytes). `field` is the transcript word field
+ * and `item_bytes` its storage width. Plain data: copying it forks the
+ * transcript, as the reference's immutable transcripts do. */
+typedef struct zs_transcript {
+  uint32_t kind;
+  uint32_t field;
+  uint32_t item_bytes;
+  uint32_t reserved;
+  zs_sha256_state sha;
+} zs_transcript;
+
+typedef struct zs_prove_args {
+  uint32_t table_field;      /* storage field of `tables` */
+  uint32_t chal_field;       /* challenge / message / claim field */
+  uint32_t mode;             /* ZS_MODE_* */
+  uint32_t degree;           /* ProductSummand degree; must equal num_tables */
+  uint32_t num_tables;       /* k stacked factor tables */
+  uint32_t rounds;           /* n: every table has 2^n entries */
+  const void* tables;        /* k x 2^n elements, table-major */
+  uint32_t tables_on_device; /* 1: `tables` is a device pointer */
+  uint32_t reserved;
+  const void* eq_point;      /* ZS_MODE_EQ: n challenge-field elements (host) */
+  const void* claim;         /* one challenge-field element (host) */
+} zs_prove_args;
+
+/* Caller-allocated host outputs. npts = degree + 1 (product) or degree + 2 (eq). */
+typedef struct zs_proof {
+  void* msgs;         /* rounds x npts challenge-field elements */
+  void* point;        /* rounds x challenge-field raw squeezed words */
+  void* value;        /* one challenge-field element */
+  void* final_tables; /* optional: k folded values (+ eq mass for ZS_MODE_EQ) */
+} zs_proof;
+
+const char* zs_last_error(void);
+const char* zs_version(void);
+int zs_device_count(int* count);
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the reference's implicit FRX device (frx.device_put / PJRT client,
+ * zorch/transcript.py:949-962): owns the stream, work buffers and budget. */
+int zs_ctx_create(int device, uint64_t hbm_budget_bytes, zs_ctx** out);
+void zs_ctx_destroy(zs_ctx* ctx);
+/* Bytes of device memory the context currently holds. */
+int zs_ctx_reserved_bytes(zs_ctx* ctx, uint64_t* out);
+
+/* ---- kernel timing -------------------------------------------------------
+ * Profiling aid with no counterpart in the reference (whose only tracing is
+ * XLA's dump flags): while enabled, every pass, finisher and tail kernel the
+ * context launches is bracketed by HIP events on the context's stream. A read
+ * returns one record per kernel with its algorithmic HBM bytes (fused-pass
+ * model) and its measured duration, then clears the list. kind: 1 = pass,
+ * 2 = finisher, 3 = tail (all remaining rounds in one block). */
+typedef struct zs_kernel_record {
+  uint32_t round;
+  uint32_t kind;
+  uint32_t grid;
+  uint32_t reserved;
+  uint64_t bytes;
+  double ms;
+} zs_kernel_record;
+int zs_ctx_profile(zs_ctx* ctx, int enable);
+int zs_ctx_profile_read(zs_ctx* ctx, zs_kernel_record* out, uint32_t cap, uint32_t* count);
+
+/* ---- transcript (host) ------------------------------------------------
+ * Replaces hash-frx's Sha256State stream under Sha256FieldTranscript
+ * (zorch/sha256_field_transcript.py:45-50,100-114,178-181). Framing is
+ * composed by the caller exactly as the reference's Python does. */
+int zs_transcript_new(uint32_t kind, uint32_t field, const uint8_t* domain, size_t len,
+                      zs_transcript* out);
+/* sha256_stream_absorb (zorch/sha256_field_transcript.py:180,196). */
+int zs_transcript_absorb(zs_transcript* t, const uint8_t* data, size_t len);
+/* _squeeze_hop minus its framing absorb: SHA256(S||le64(ctr)) blocks,
+ * truncated to nbytes, re-absorbed (zorch/sha256_field_transcript.py:100-114). */
+int zs_transcript_squeeze(zs_transcript* t, size_t nbytes, uint8_t* out);
+/* SHA256 of the stream so far (the proof-of-work state digest,
+ * zorch/sha256_field_transcript.py:277-284). */
+int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]);
+
+/* ---- prover ------------------------------------------------------------
+ * Whole stage on the device: SumcheckProver.prove / EqPolyProver.prove
+ * (zorch/sumcheck/stage.py:52-69, zorch/sumcheck/eq/stage.py:62-92) driving
+ * fold_rounds (zorch/prove.py:26-37). One fused fold+evaluate pass per round,
+ * the Fiat-Shamir hop on the device; `transcript` is read and advanced. */
+int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
+             zs_proof* out);
+/* Many independent instances (uneven sizes) in one submission; instance i
+ * uses args[i], transcripts[i], outs[i]. */
+int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
+                   zs_transcript* transcripts, zs_proof* outs);
+
+/* ---- per-round state (drop-in under fold_rounds) -----------------------
+ * initial_claim (zorch/sumcheck/prover.py:86-91) on device-resident tables,
+ * then one ProverRound step per call: StandardRound.__call__
+ * (zorch/sumcheck/prover.py:157-174) or EqPolyRound.__call__
+ * (zorch/sumcheck/eq/eq_poly.py:214-221). */
+int zs_state_create(zs_ctx* ctx, const zs_prove_args* args, zs_state** out);
+/* Advance one round: writes the message (npts elements), the raw challenge
+ * and the reduced claim; advances `transcript`. */
+int zs_state_round(zs_state* st, zs_transcript* transcript, void* msg_out, void* chal_out,
+                   void* claim_out);
+/* Rounds already bound, and a copy of the current folded tables
+ * (k x 2^(n-bound) challenge-field elements; before round 1 the tables are
+ * returned lifted into the challenge field). */
+int zs_state_bound(const zs_state* st, uint32_t* bound);
+int zs_state_read_tables(zs_state* st, void* out, size_t out_bytes);
+void zs_state_destroy(zs_state* st);
+
+/* ---- verifier (host) ---------------------------------------------------
+ * One SumcheckRound step (zorch/sumcheck/verifier.py:62-71): observe(msg),
+ * sample the challenge, reduce_evals (zorch/sumcheck/reduce.py:27-32).
+ * Writes the raw challenge words, the reduced claim and ok (1/0). Width
+ * mismatch -> ZS_EINVAL. */
+int zs_verify_round(uint32_t chal_field, zs_transcript* transcript, const void* claim,
+                    const void* msg, uint32_t npts, uint32_t degree, void* chal_out,
+                    void* reduced_out, int* ok);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZORCH_AMD_H_ */

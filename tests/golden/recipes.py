@@ -1,0 +1,44 @@
+"""Instance recipes of the golden fixtures (seeds and sizes only)."""
+
+from __future__ import annotations
+
+from zorch_amd import workloads as W
+from zorch_amd.fields import babybear_mont, babybearx4_mont
+
+RECIPES = {
+    # The plain product config at its full size: BabyBear, degree 2, n = 16.
+    "product_bb_n16": {"kind": "product_bb", "n": 16, "seeds": [1, 2], "label": 0},
+    "product_bb_n4": {"kind": "product_bb", "n": 4, "seeds": [1, 2], "label": 0},
+    "product_bb_k1_n5": {"kind": "product_bb", "n": 5, "seeds": [7], "label": 1},
+    "product_bb_k3_n4": {"kind": "product_bb", "n": 4, "seeds": [7, 8, 9], "label": 2},
+    "product_bb_n1": {"kind": "product_bb", "n": 1, "seeds": [3, 4], "label": 3},
+    # Product with extension challenges (tables lifted into BabyBear^4).
+    "product_bb_bb4_n6": {"kind": "product_bb_ext", "n": 6, "seeds": [5, 6], "label": 4},
+    # Zerocheck eq(z,x) A B over BabyBear^4 challenges, at reduced n.
+    "zerocheck_bb4_n2": {"kind": "zerocheck", "n": 2, "seeds": [1, 2], "z": 3, "label": 0},
+    "zerocheck_bb4_n5": {"kind": "zerocheck", "n": 5, "seeds": [1, 2], "z": 3, "label": 0},
+    "zerocheck_bb4_n9": {"kind": "zerocheck", "n": 9, "seeds": [1, 2], "z": 3, "label": 0},
+    "zerocheck_bb4_k1_n4": {"kind": "zerocheck", "n": 4, "seeds": [11], "z": 12, "label": 5},
+    "zerocheck_bb4_k3_n4": {"kind": "zerocheck", "n": 4, "seeds": [11, 12, 13], "z": 14,
+                            "label": 6},
+    # The same with a BabyBear^4 transcript (one word per challenge).
+    "zerocheck_bb4_t4_n5": {"kind": "zerocheck", "n": 5, "seeds": [1, 2], "z": 3, "label": 0,
+                            "transcript": "babybearx4_mont"},
+    # Goldilocks product at reduced n.
+    "product_gl_n5": {"kind": "product_gl", "n": 5, "seeds": [1, 2], "label": 0},
+    "product_gl_n10": {"kind": "product_gl", "n": 10, "seeds": [1, 2], "label": 0},
+}
+
+
+def build(recipe: dict) -> W.Instance:
+    kind, n, seeds, lab = recipe["kind"], recipe["n"], tuple(recipe["seeds"]), recipe["label"]
+    if kind == "product_bb":
+        return W.product_bb(n, seeds, lab)
+    if kind == "product_bb_ext":
+        return W.product_bb_ext(n, seeds, lab)
+    if kind == "product_gl":
+        return W.product_gl(n, seeds, lab)
+    if kind == "zerocheck":
+        tf = babybearx4_mont if recipe.get("transcript") == "babybearx4_mont" else babybear_mont
+        return W.zerocheck_bb4(n, seeds, recipe["z"], lab, transcript_field=tf)
+    raise ValueError(kind)

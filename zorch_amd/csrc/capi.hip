@@ -1,0 +1,431 @@
+// C ABI of the prover: context, host transcript, whole-stage and per-round
+// proving, batched proving and the host verifier step.
+#include <string.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/zorch_amd.h"
+#include "field.h"
+#include "runtime.h"
+#include "sha256.h"
+
+namespace zs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? ZS_ENOMEM : ZS_EHIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------- pool
+DevicePool::~DevicePool() {
+  trim();
+  for (auto& kv : sizes_) (void)hipFree(kv.first);
+}
+
+void DevicePool::trim() {
+  for (auto& kv : free_) {
+    (void)hipFree(kv.second);
+    sizes_.erase(kv.second);
+  }
+  free_.clear();
+  cached_ = 0;
+}
+
+int DevicePool::alloc(size_t bytes, void** out) {
+  bytes = (bytes + 4095) & ~(size_t)4095;
+  auto it = free_.find(bytes);
+  if (it != free_.end()) {
+    *out = it->second;
+    free_.erase(it);
+    cached_ -= bytes;
+    live_ += bytes;
+    return ZS_OK;
+  }
+  if (live_ + bytes > budget_) {
+    return fail(ZS_EBUDGET, "device memory budget exceeded: " + std::to_string(live_ + bytes) +
+                                " > " + std::to_string(budget_) + " bytes");
+  }
+  if (live_ + cached_ + bytes > budget_) trim();
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    trim();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  }
+  sizes_[p] = bytes;
+  live_ += bytes;
+  *out = p;
+  return ZS_OK;
+}
+
+void DevicePool::release(void* p) {
+  auto it = sizes_.find(p);
+  if (it == sizes_.end()) return;
+  live_ -= it->second;
+  cached_ += it->second;
+  free_.emplace(it->second, p);
+}
+
+// ---------------------------------------------------------------- helpers
+uint32_t field_bytes(uint32_t field) {
+  switch (field) {
+    case ZS_F_BABYBEAR: return 4;
+    case ZS_F_BABYBEAR4: return 16;
+    case ZS_F_GOLDILOCKS: return 8;
+    default: return 0;
+  }
+}
+
+// Base prime field of a field id.
+static uint32_t base_of(uint32_t f) { return f == ZS_F_BABYBEAR4 ? ZS_F_BABYBEAR : f; }
+
+int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts) {
+  if (!field_bytes(a.table_field) || !field_bytes(a.chal_field))
+    return fail(ZS_EINVAL, "unknown field id");
+  const bool ok_pair = (a.table_field == a.chal_field) ||
+                       (a.table_field == ZS_F_BABYBEAR && a.chal_field == ZS_F_BABYBEAR4);
+  if (!ok_pair) return fail(ZS_EINVAL, "table field does not embed in the challenge field");
+  if (a.mode != ZS_MODE_PRODUCT && a.mode != ZS_MODE_EQ) return fail(ZS_EINVAL, "unknown mode");
+  if (a.degree < 1) return fail(ZS_EINVAL, "degree must be >= 1");
+  if (a.num_tables < 1 || a.num_tables > 4)
+    return fail(ZS_EINVAL, "between 1 and 4 stacked factor tables are supported");
+  if (a.degree != a.num_tables)
+    return fail(ZS_EINVAL, "the product summand's degree must equal the number of stacked tables");
+  if (a.rounds < 1 || a.rounds > 33) return fail(ZS_EINVAL, "rounds must be in [1, 33]");
+  if (!a.tables || !a.claim) return fail(ZS_EINVAL, "null tables or claim");
+  if (a.mode == ZS_MODE_EQ && !a.eq_point) return fail(ZS_EINVAL, "eq mode needs an equality point");
+  if (t) {
+    if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+    const uint32_t tf = t->field;
+    if (!(tf == a.chal_field || tf == base_of(a.chal_field)))
+      return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
+    if (t->item_bytes != field_bytes(tf)) return fail(ZS_EINVAL, "transcript item size mismatch");
+  }
+  *npts = a.degree + 1 + (a.mode == ZS_MODE_EQ ? 1 : 0);
+  return ZS_OK;
+}
+
+Instance* make_instance(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_instance_bb(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_instance_bb_bb4(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_instance_bb4(k, npts, eq);
+  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_instance_gl(k, npts, eq);
+  return nullptr;
+}
+
+static int set_device(zs_ctx* ctx) {
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return ZS_OK;
+}
+
+// ------------------------------------------------------------- verifier
+template <class F>
+static int verify_round_typed(zs_transcript* t, const void* claim_p, const void* msg_p,
+                              uint32_t npts, void* chal_out, void* reduced_out, int* ok) {
+  using T = typename F::T;
+  std::vector<T> msg(npts);
+  memcpy(msg.data(), msg_p, npts * sizeof(T));
+  T claim;
+  memcpy(&claim, claim_p, sizeof(T));
+  claim = F::canon(claim);
+  for (auto& m : msg) m = F::canon(m);
+  const size_t item = t->item_bytes;
+  const size_t limbs = sizeof(T) / item;
+  std::vector<uint8_t> raw(sizeof(T));
+  Sha256State* s = reinterpret_cast<Sha256State*>(&t->sha);
+  sha256_fs_hop(*s, reinterpret_cast<const uint8_t*>(msg_p), npts * sizeof(T), item, limbs,
+                raw.data());
+  if (chal_out) memcpy(chal_out, raw.data(), sizeof(T));
+  T r;
+  memcpy(&r, raw.data(), sizeof(T));
+  r = F::canon(r);
+  // Lagrange over {0..npts-1} at r (reference poly/univariate.py:36-65).
+  T red = F::zero();
+  for (uint32_t k = 0; k < npts; ++k) {
+    T num = F::one(), den = F::one();
+    for (uint32_t j = 0; j < npts; ++j) {
+      if (j == k) continue;
+      num = F::mul(num, F::sub(r, F::from_int(j)));
+      den = F::mul(den, F::from_int((int64_t)k - (int64_t)j));
+    }
+    red = F::add(red, F::mul(msg[k], F::mul(num, F::inv(den))));
+  }
+  if (reduced_out) memcpy(reduced_out, &red, sizeof(T));
+  *ok = F::eq(claim, F::add(msg[0], msg[1])) ? 1 : 0;
+  return ZS_OK;
+}
+
+}  // namespace zs
+
+using namespace zs;
+
+struct zs_state {
+  zs_ctx* ctx = nullptr;
+  std::unique_ptr<Instance> inst;
+  uint32_t bound = 0;
+};
+
+extern "C" {
+
+const char* zs_last_error(void) { return g_last_error.c_str(); }
+const char* zs_version(void) { return "zorch_amd 0.1.0 gfx950"; }
+
+int zs_device_count(int* count) {
+  if (!count) return fail(ZS_EINVAL, "null out");
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return hip_fail(e, "hipGetDeviceCount");
+  }
+  return ZS_OK;
+}
+
+int zs_ctx_create(int device, uint64_t budget, zs_ctx** out) {
+  if (!out) return fail(ZS_EINVAL, "null out");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(ZS_ENODEV, "no HIP device available");
+  if (device < 0 || device >= count) return fail(ZS_EINVAL, "device index out of range");
+  ZS_HIP(hipSetDevice(device));
+  auto* ctx = new zs_ctx();
+  ctx->device = device;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    ZS_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget = (uint64_t)(free_b * 0.9);
+  }
+  ctx->pool = new DevicePool(budget);
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx->pool;
+    delete ctx;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = ctx;
+  return ZS_OK;
+}
+
+void zs_ctx_destroy(zs_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->records) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  delete ctx->pool;
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int zs_ctx_reserved_bytes(zs_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
+  *out = ctx->pool->held();
+  return ZS_OK;
+}
+
+int zs_ctx_profile(zs_ctx* ctx, int enable) {
+  if (!ctx) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->records) {
+    ctx->spare_events.push_back(r.start);
+    ctx->spare_events.push_back(r.stop);
+  }
+  ctx->records.clear();
+  ctx->profile = enable != 0;
+  return ZS_OK;
+}
+
+int zs_ctx_profile_read(zs_ctx* ctx, zs_kernel_record* out, uint32_t cap, uint32_t* count) {
+  if (!ctx || !count || (cap && !out)) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t n = 0;
+  for (auto& r : ctx->records) {
+    if (n < cap) {
+      float ms = 0.f;
+      ZS_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+      out[n] = zs_kernel_record{r.round, r.kind, r.grid, 0, r.bytes, (double)ms};
+    }
+    ++n;
+    ctx->spare_events.push_back(r.start);
+    ctx->spare_events.push_back(r.stop);
+  }
+  ctx->records.clear();
+  *count = n < cap ? n : cap;
+  return ZS_OK;
+}
+
+// ------------------------------------------------------------- transcript
+int zs_transcript_new(uint32_t kind, uint32_t field, const uint8_t* domain, size_t len,
+                      zs_transcript* out) {
+  if (!out || (len && !domain)) return fail(ZS_EINVAL, "null argument");
+  if (kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (!field_bytes(field)) return fail(ZS_EINVAL, "unknown field id");
+  memset(out, 0, sizeof(*out));
+  out->kind = kind;
+  out->field = field;
+  out->item_bytes = field_bytes(field);
+  Sha256State* s = reinterpret_cast<Sha256State*>(&out->sha);
+  sha256_init(*s);
+  sha256_absorb_byte(*s, OP_DOMAIN);
+  sha256_absorb_le64(*s, len);
+  sha256_absorb(*s, domain, len);
+  return ZS_OK;
+}
+
+int zs_transcript_absorb(zs_transcript* t, const uint8_t* data, size_t len) {
+  if (!t || (len && !data)) return fail(ZS_EINVAL, "null argument");
+  sha256_absorb(*reinterpret_cast<Sha256State*>(&t->sha), data, len);
+  return ZS_OK;
+}
+
+int zs_transcript_squeeze(zs_transcript* t, size_t nbytes, uint8_t* out) {
+  if (!t || (nbytes && !out)) return fail(ZS_EINVAL, "null argument");
+  sha256_squeeze(*reinterpret_cast<Sha256State*>(&t->sha), out, nbytes);
+  return ZS_OK;
+}
+
+int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]) {
+  if (!t || !out) return fail(ZS_EINVAL, "null argument");
+  sha256_digest_suffix(*reinterpret_cast<const Sha256State*>(&t->sha), nullptr, 0, out);
+  return ZS_OK;
+}
+
+// ------------------------------------------------------------------ prove
+int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript, zs_proof* out) {
+  if (!ctx || !args || !transcript || !out) return fail(ZS_EINVAL, "null argument");
+  uint32_t npts = 0;
+  int rc = validate_args(*args, transcript, &npts);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  std::unique_ptr<Instance> inst(
+      make_instance(args->table_field, args->chal_field, args->num_tables, npts,
+                    args->mode == ZS_MODE_EQ));
+  if (!inst) return fail(ZS_EINVAL, "unsupported field / degree combination");
+  if ((rc = inst->setup(ctx, *args, *transcript))) return rc;
+  if ((rc = inst->enqueue(0, args->rounds, ctx->stream))) return rc;
+  if ((rc = inst->download(ctx->stream))) return rc;
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  return inst->collect(out, transcript);
+}
+
+int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
+                   zs_transcript* transcripts, zs_proof* outs) {
+  if (!ctx || (count && (!args || !transcripts || !outs))) return fail(ZS_EINVAL, "null argument");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  std::vector<std::unique_ptr<Instance>> insts(count);
+  for (size_t i = 0; i < count; ++i) {
+    uint32_t npts = 0;
+    if ((rc = validate_args(args[i], &transcripts[i], &npts))) return rc;
+    insts[i].reset(make_instance(args[i].table_field, args[i].chal_field, args[i].num_tables,
+                                 npts, args[i].mode == ZS_MODE_EQ));
+    if (!insts[i]) return fail(ZS_EINVAL, "unsupported field / degree combination");
+    if ((rc = insts[i]->setup(ctx, args[i], transcripts[i]))) return rc;
+  }
+  for (size_t i = 0; i < count; ++i) {
+    if ((rc = insts[i]->enqueue(0, args[i].rounds, ctx->stream))) return rc;
+    if ((rc = insts[i]->download(ctx->stream))) return rc;
+  }
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < count; ++i)
+    if ((rc = insts[i]->collect(&outs[i], &transcripts[i]))) return rc;
+  return ZS_OK;
+}
+
+// ------------------------------------------------------------ per round
+int zs_state_create(zs_ctx* ctx, const zs_prove_args* args, zs_state** out) {
+  if (!ctx || !args || !out) return fail(ZS_EINVAL, "null argument");
+  uint32_t npts = 0;
+  int rc = validate_args(*args, nullptr, &npts);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  auto st = std::make_unique<zs_state>();
+  st->ctx = ctx;
+  st->inst.reset(make_instance(args->table_field, args->chal_field, args->num_tables, npts,
+                               args->mode == ZS_MODE_EQ));
+  if (!st->inst) return fail(ZS_EINVAL, "unsupported field / degree combination");
+  zs_transcript blank;
+  memset(&blank, 0, sizeof(blank));
+  blank.item_bytes = field_bytes(base_of(args->chal_field));
+  if ((rc = st->inst->setup(ctx, *args, blank))) return rc;
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  *out = st.release();
+  return ZS_OK;
+}
+
+int zs_state_round(zs_state* st, zs_transcript* t, void* msg_out, void* chal_out,
+                   void* claim_out) {
+  if (!st || !t) return fail(ZS_EINVAL, "null argument");
+  if (st->bound >= st->inst->rounds) return fail(ZS_EINVAL, "every round is already bound");
+  if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (t->item_bytes == 0 || st->inst->chal_bytes % t->item_bytes)
+    return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
+  int rc;
+  if ((rc = set_device(st->ctx))) return rc;
+  hipStream_t s = st->ctx->stream;
+  if ((rc = st->inst->upload_transcript(*t, s))) return rc;
+  if ((rc = st->inst->enqueue(st->bound, st->bound + 1, s))) return rc;
+  if ((rc = st->inst->download_round(st->bound, s))) return rc;
+  ZS_HIP(hipStreamSynchronize(s));
+  if ((rc = st->inst->collect_round(st->bound, msg_out, chal_out, claim_out, t))) return rc;
+  st->bound += 1;
+  return ZS_OK;
+}
+
+int zs_state_bound(const zs_state* st, uint32_t* bound) {
+  if (!st || !bound) return fail(ZS_EINVAL, "null argument");
+  *bound = st->bound;
+  return ZS_OK;
+}
+
+int zs_state_read_tables(zs_state* st, void* out, size_t out_bytes) {
+  if (!st || !out) return fail(ZS_EINVAL, "null argument");
+  int rc;
+  if ((rc = set_device(st->ctx))) return rc;
+  return st->inst->read_tables(st->bound, out, out_bytes, st->ctx->stream);
+}
+
+void zs_state_destroy(zs_state* st) {
+  if (!st) return;
+  (void)hipSetDevice(st->ctx->device);
+  (void)hipStreamSynchronize(st->ctx->stream);
+  delete st;
+}
+
+int zs_verify_round(uint32_t chal_field, zs_transcript* t, const void* claim, const void* msg,
+                    uint32_t npts, uint32_t degree, void* chal_out, void* reduced_out, int* ok) {
+  if (!t || !claim || !msg || !ok) return fail(ZS_EINVAL, "null argument");
+  if (degree < 1) return fail(ZS_EINVAL, "degree must be >= 1");
+  if (npts != degree + 1)
+    return fail(ZS_EINVAL, "round message must have " + std::to_string(degree + 1) +
+                               " evals, got " + std::to_string(npts));
+  if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  const uint32_t cb = field_bytes(chal_field);
+  if (!cb) return fail(ZS_EINVAL, "unknown field id");
+  if (!(t->field == chal_field || t->field == base_of(chal_field)) || t->item_bytes == 0 ||
+      cb % t->item_bytes)
+    return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
+  switch (chal_field) {
+    case ZS_F_BABYBEAR: return verify_round_typed<BB>(t, claim, msg, npts, chal_out, reduced_out, ok);
+    case ZS_F_BABYBEAR4: return verify_round_typed<BB4>(t, claim, msg, npts, chal_out, reduced_out, ok);
+    case ZS_F_GOLDILOCKS: return verify_round_typed<GL>(t, claim, msg, npts, chal_out, reduced_out, ok);
+  }
+  return fail(ZS_EINVAL, "unknown field id");
+}
+
+}  // extern "C"

@@ -1,0 +1,8 @@
+// Proof instances with BB4 tables and BB4 challenges.
+#include "instance.h"
+
+namespace zs {
+Instance* make_instance_bb4(uint32_t k, uint32_t npts, bool eq) {
+  return make_typed<BB4, BB4>(k, npts, eq);
+}
+}  // namespace zs

@@ -1,0 +1,132 @@
+// Host runtime of the prover: error reporting, the per-device context (one
+// HIP stream, a budgeted device-memory pool) and the type-erased proof
+// instance the C ABI drives.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/zorch_amd.h"
+
+namespace zs {
+
+// Thread-local message for the last failing call.
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define ZS_HIP(call)                                    \
+  do {                                                  \
+    hipError_t zs_e_ = (call);                          \
+    if (zs_e_ != hipSuccess) return hip_fail(zs_e_, #call); \
+  } while (0)
+
+// Budgeted device-memory pool. Blocks are cached by exact size and reused;
+// `budget` bounds live + cached bytes (cached blocks are released first when
+// a new allocation would exceed it).
+class DevicePool {
+ public:
+  explicit DevicePool(uint64_t budget) : budget_(budget) {}
+  ~DevicePool();
+  int alloc(size_t bytes, void** out);
+  void release(void* p);
+  void trim();
+  uint64_t held() const { return live_ + cached_; }
+  uint64_t live() const { return live_; }
+  uint64_t budget() const { return budget_; }
+
+ private:
+  uint64_t budget_;
+  uint64_t live_ = 0, cached_ = 0;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> sizes_;
+};
+
+struct TuneParams {
+  uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
+  uint32_t max_grid = 2048;     // blocks of a multi-block pass (<= MAXG)
+  uint32_t pairs_per_thread = 4;  // target pairs per thread of a multi-block pass
+};
+
+// Kernel timing records (enabled by zs_ctx_profile).
+struct KernelRecord {
+  uint32_t round, kind, grid;
+  uint64_t bytes;
+  hipEvent_t start, stop;
+};
+enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3 };
+
+}  // namespace zs
+
+struct zs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  zs::DevicePool* pool = nullptr;
+  zs::TuneParams tune;
+  bool profile = false;
+  std::vector<zs::KernelRecord> records;
+  std::vector<hipEvent_t> spare_events;
+  hipEvent_t event() {
+    if (!spare_events.empty()) {
+      hipEvent_t e = spare_events.back();
+      spare_events.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+
+namespace zs {
+
+// Device buffers of one proof instance, carved from one pool block.
+struct InstanceBuffers {
+  void* block = nullptr;
+  size_t block_bytes = 0;
+};
+
+// One proof in flight (type-erased over field / factor count / mode).
+class Instance {
+ public:
+  virtual ~Instance() = default;
+  // Allocate, upload inputs and the transcript, build eq tables.
+  virtual int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t) = 0;
+  // Enqueue rounds [first, last) on `stream`.
+  virtual int enqueue(uint32_t first, uint32_t last, hipStream_t stream) = 0;
+  // Enqueue the copy of the outputs to pinned host staging (whole proof).
+  virtual int download(hipStream_t stream) = 0;
+  // After the stream has synchronised: copy staged outputs to the caller.
+  virtual int collect(zs_proof* out, zs_transcript* t) = 0;
+  // Per-round API: upload a transcript state before a round; read one round.
+  virtual int upload_transcript(const zs_transcript& t, hipStream_t stream) = 0;
+  virtual int download_round(uint32_t round, hipStream_t stream) = 0;
+  virtual int collect_round(uint32_t round, void* msg, void* chal, void* claim,
+                            zs_transcript* t) = 0;
+  virtual int read_tables(uint32_t bound, void* out, size_t out_bytes, hipStream_t stream) = 0;
+  virtual void free_buffers() = 0;
+  uint32_t rounds = 0;
+  uint32_t npts = 0;
+  uint32_t chal_bytes = 0;
+  uint32_t k = 0;
+  uint64_t device_bytes = 0;
+};
+
+// Factory: returns nullptr for an unsupported combination.
+Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
+                        bool eq);
+// Per-field-combination factories (one translation unit each).
+Instance* make_instance_bb(uint32_t k, uint32_t npts, bool eq);
+Instance* make_instance_bb_bb4(uint32_t k, uint32_t npts, bool eq);
+Instance* make_instance_bb4(uint32_t k, uint32_t npts, bool eq);
+Instance* make_instance_gl(uint32_t k, uint32_t npts, bool eq);
+
+uint32_t field_bytes(uint32_t field);
+int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts);
+
+}  // namespace zs

@@ -1,0 +1,203 @@
+"""Device context and device-resident folding state.
+
+``Context`` wraps a ``zs_ctx`` (one HIP stream and a budgeted device memory
+pool on one GPU). ``DeviceFoldState`` is the folding state a sumcheck round
+threads through ``fold_rounds``: the stacked tables live on the device and are
+folded there; each round advances it once.
+
+There is no host fallback: creating a context without a GPU raises
+``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import Any
+
+import numpy as np
+
+from zorch_amd import _lib
+from zorch_amd.fields import Field, FieldArray, as_field, from_bytes
+
+
+class Context:
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().zs_ctx_create(device, hbm_budget_bytes, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError("context is closed")
+        return self._h
+
+    def reserved_bytes(self) -> int:
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().zs_ctx_reserved_bytes(self.handle, ctypes.byref(out)))
+        return int(out.value)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().zs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default: Context | None = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    """The process-wide context on device 0 (created on first use)."""
+    global _default
+    if _default is None:
+        with _default_lock:
+            if _default is None:
+                _default = Context(0)
+    return _default
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    rc = _lib.lib().zs_device_count(ctypes.byref(n))
+    return int(n.value) if rc == 0 else 0
+
+
+class DeviceTables:
+    """Stacked factor tables already resident on the GPU.
+
+    `tensor` is a torch CUDA tensor holding the storage words (any integer
+    dtype of the right width), shape (k, 2^n, words) or (k, 2^n) for prime
+    fields. Used to hand the prover inputs that are already in HBM.
+    """
+
+    def __init__(self, tensor: Any, field: Any):
+        self.tensor = tensor
+        self.field = as_field(field)
+        shape = tuple(tensor.shape)
+        if self.field.words > 1 or (len(shape) == 3 and shape[-1] == 1):
+            shape = shape[:-1]
+        self.shape = shape
+        if len(self.shape) != 2:
+            raise ValueError("device tables must be (k, 2^n)")
+        if not tensor.is_cuda or not tensor.is_contiguous():
+            raise ValueError("device tables must be a contiguous CUDA tensor")
+
+    @property
+    def dtype(self) -> Field:
+        return self.field
+
+    def data_ptr(self) -> int:
+        return int(self.tensor.data_ptr())
+
+
+def tables_pointer(state) -> tuple[int, bool, Any, tuple[int, ...]]:
+    """(pointer, on_device, keepalive, shape) of a stacked table argument."""
+    if isinstance(state, DeviceTables):
+        return state.data_ptr(), True, state, state.shape
+    if isinstance(state, FieldArray):
+        words = np.ascontiguousarray(state.words)
+        return words.ctypes.data, False, words, state.shape
+    raise TypeError("tables must be a FieldArray or DeviceTables")
+
+
+def log2_strict(n: int) -> int:
+    if n <= 0 or n & (n - 1):
+        raise ValueError(f"{n} is not a power of two")
+    return n.bit_length() - 1
+
+
+def el_buffer(value: FieldArray, field: Field):
+    """Contiguous storage words of `value` lifted into `field` (kept alive by the caller)."""
+    return np.ascontiguousarray(value.astype(field).words)
+
+
+class DeviceFoldState:
+    """Device-resident folding state of one sumcheck (a ``zs_state``).
+
+    Rounds advance it in place; ``bound`` counts the rounds done. A carry that
+    refers to an earlier ``bound`` has been consumed and cannot be replayed.
+    """
+
+    def __init__(self, tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
+                 eq_point: FieldArray | None = None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        ptr, on_dev, keep, shape = tables_pointer(tables)
+        self.k, width = shape
+        self.rounds = log2_strict(width)
+        self.table_field = tables.dtype if isinstance(tables, DeviceTables) else tables.field
+        self.chal_field = as_field(chal_field)
+        self.mode = mode
+        self.degree = degree
+        self.npts = degree + 1 + (1 if mode == _lib.ZS_MODE_EQ else 0)
+        claim_buf = el_buffer(claim, self.chal_field)
+        eq_buf = el_buffer(eq_point, self.chal_field) if eq_point is not None else None
+        args = _lib.ProveArgsC(
+            table_field=self.table_field.fid, chal_field=self.chal_field.fid, mode=mode,
+            degree=degree, num_tables=self.k, rounds=self.rounds, tables=ptr,
+            tables_on_device=1 if on_dev else 0, reserved=0,
+            eq_point=eq_buf.ctypes.data if eq_buf is not None else None,
+            claim=claim_buf.ctypes.data)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().zs_state_create(self.ctx.handle, ctypes.byref(args), ctypes.byref(h)))
+        self._h = h
+        self._keep = keep
+        self.bound = 0
+        self.eq_point = eq_point
+
+    def round(self, transcript):
+        """One round: returns (msg, raw challenge, reduced claim, new transcript)."""
+        f = self.chal_field
+        msg = np.zeros((self.npts, f.words), dtype=f.word_dtype)
+        chal = np.zeros((f.words,), dtype=f.word_dtype)
+        claim = np.zeros((f.words,), dtype=f.word_dtype)
+        st = transcript.state.copy()
+        _lib.check(_lib.lib().zs_state_round(self._h, ctypes.byref(st), msg.ctypes.data,
+                                             chal.ctypes.data, claim.ctypes.data))
+        self.bound += 1
+        new_t = type(transcript)(st, transcript.dtype)
+        return FieldArray(msg, f), FieldArray(chal, f), FieldArray(claim, f), new_t
+
+    def read_tables(self) -> FieldArray:
+        """Current folded tables (k, 2^(n - bound)) in the challenge field."""
+        f = self.chal_field
+        length = 1 << (self.rounds - self.bound)
+        out = np.zeros((self.k, length, f.words), dtype=f.word_dtype)
+        _lib.check(_lib.lib().zs_state_read_tables(self._h, out.ctypes.data, out.nbytes))
+        return FieldArray(out, f)
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.k, 1 << (self.rounds - self.bound))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None:
+            try:
+                _lib.lib().zs_state_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def profile(ctx: Context, enable: bool) -> None:
+    """Turn per-kernel event timing on or off (clears previous records)."""
+    _lib.check(_lib.lib().zs_ctx_profile(ctx.handle, 1 if enable else 0))
+
+
+def profile_read(ctx: Context, cap: int = 1 << 16) -> list[dict]:
+    """Kernel records since the last read: round, kind, grid, algorithmic bytes, ms."""
+    recs = (_lib.KernelRecordC * cap)()
+    n = ctypes.c_uint32(0)
+    _lib.check(_lib.lib().zs_ctx_profile_read(ctx.handle, recs, cap, ctypes.byref(n)))
+    kinds = {1: "pass", 2: "finish", 3: "tail"}
+    return [{"round": r.round, "kind": kinds.get(r.kind, str(r.kind)), "grid": r.grid,
+             "bytes": int(r.bytes), "ms": float(r.ms)} for r in recs[: n.value]]
