@@ -99,6 +99,16 @@ void zs_ctx_destroy(zs_ctx* ctx);
 /* Bytes of device memory the context currently holds. */
 int zs_ctx_reserved_bytes(zs_ctx* ctx, uint64_t* out);
 
+/* Device buffers from the context's budgeted pool, for inputs that should be
+ * resident in HBM before proving (tables_on_device = 1). The reference keeps
+ * arrays on its FRX device the same way (frx.device_put,
+ * zorch/transcript.py:949-962). Copies are synchronous on the context's
+ * stream. */
+int zs_device_alloc(zs_ctx* ctx, size_t bytes, void** dptr);
+int zs_device_free(zs_ctx* ctx, void* dptr);
+int zs_memcpy_htod(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
+int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 /* ---- kernel timing -------------------------------------------------------
  * Profiling aid with no counterpart in the reference (whose only tracing is
  * XLA's dump flags): while enabled, every pass, finisher and tail kernel the

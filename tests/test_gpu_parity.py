@@ -161,14 +161,15 @@ def test_product_bb_n16_full_size_and_determinism():
 
 
 def test_device_resident_tables_equal_host_tables():
-    import torch
     from zorch_amd.runtime import DeviceTables
     inst = W.zerocheck_bb4(14, (81, 82), 83, 9)
-    dev = torch.from_numpy(inst.tables.words.view(np.int32).copy()).cuda()
+    dev = DeviceTables.from_host(inst.tables)
+    assert dev.to_host().words.tolist() == inst.tables.words.tolist()
     a = _prove(inst)
-    b = _prove(inst, DeviceTables(dev, babybear_mont))
+    b = _prove(inst, dev)
     assert a.reduction_proof.words.tolist() == b.reduction_proof.words.tolist()
     assert a.reduced_claim.point.words.tolist() == b.reduced_claim.point.words.tolist()
+    dev.free()
 
 
 def test_structural_errors_raise_value_error():

@@ -96,6 +96,10 @@ SIGNATURES = {
     "zs_ctx_create": (_INT, [_INT, _U64, ctypes.POINTER(_P)]),
     "zs_ctx_destroy": (None, [_P]),
     "zs_ctx_reserved_bytes": (_INT, [_P, ctypes.POINTER(_U64)]),
+    "zs_device_alloc": (_INT, [_P, _SZ, ctypes.POINTER(_P)]),
+    "zs_device_free": (_INT, [_P, _P]),
+    "zs_memcpy_htod": (_INT, [_P, _P, _P, _SZ]),
+    "zs_memcpy_dtoh": (_INT, [_P, _P, _P, _SZ]),
     "zs_ctx_profile": (_INT, [_P, _INT]),
     "zs_ctx_profile_read": (_INT, [_P, ctypes.POINTER(KernelRecordC), _U32,
                                    ctypes.POINTER(_U32)]),

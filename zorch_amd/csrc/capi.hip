@@ -236,6 +236,36 @@ int zs_ctx_reserved_bytes(zs_ctx* ctx, uint64_t* out) {
   return ZS_OK;
 }
 
+int zs_device_alloc(zs_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr || bytes == 0) return fail(ZS_EINVAL, "bad device allocation request");
+  ZS_HIP(hipSetDevice(ctx->device));
+  return ctx->pool->alloc(bytes, dptr);
+}
+
+int zs_device_free(zs_ctx* ctx, void* dptr) {
+  if (!ctx || !dptr) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->pool->release(dptr);
+  return ZS_OK;
+}
+
+int zs_memcpy_htod(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  return ZS_OK;
+}
+
+int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  return ZS_OK;
+}
+
 int zs_ctx_profile(zs_ctx* ctx, int enable) {
   if (!ctx) return fail(ZS_EINVAL, "null argument");
   ZS_HIP(hipSetDevice(ctx->device));

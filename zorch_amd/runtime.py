@@ -72,31 +72,58 @@ def device_count() -> int:
 
 
 class DeviceTables:
-    """Stacked factor tables already resident on the GPU.
-
-    `tensor` is a torch CUDA tensor holding the storage words (any integer
-    dtype of the right width), shape (k, 2^n, words) or (k, 2^n) for prime
-    fields. Used to hand the prover inputs that are already in HBM.
+    """Stacked factor tables resident in HBM (a device buffer of the context's
+    pool), shape (k, 2^n), in storage encoding. Hands the prover inputs that
+    are already on the GPU, so proving starts with no host-to-device copy.
     """
 
-    def __init__(self, tensor: Any, field: Any):
-        self.tensor = tensor
+    def __init__(self, ptr: int, shape: tuple[int, int], field: Any, ctx: Context,
+                 owned: bool = True):
+        self._ptr = ptr
+        self.shape = tuple(shape)
         self.field = as_field(field)
-        shape = tuple(tensor.shape)
-        if self.field.words > 1 or (len(shape) == 3 and shape[-1] == 1):
-            shape = shape[:-1]
-        self.shape = shape
+        self.ctx = ctx
+        self._owned = owned
         if len(self.shape) != 2:
             raise ValueError("device tables must be (k, 2^n)")
-        if not tensor.is_cuda or not tensor.is_contiguous():
-            raise ValueError("device tables must be a contiguous CUDA tensor")
+
+    @classmethod
+    def from_host(cls, tables: FieldArray, ctx: Context | None = None) -> "DeviceTables":
+        ctx = ctx or default_context()
+        if tables.ndim != 2:
+            raise ValueError("device tables must be (k, 2^n)")
+        words = np.ascontiguousarray(tables.words)
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().zs_device_alloc(ctx.handle, words.nbytes, ctypes.byref(p)))
+        _lib.check(_lib.lib().zs_memcpy_htod(ctx.handle, p, words.ctypes.data, words.nbytes))
+        return cls(int(p.value), tables.shape, tables.field, ctx)
+
+    def to_host(self) -> FieldArray:
+        f = self.field
+        out = np.zeros(self.shape + (f.words,), dtype=f.word_dtype)
+        _lib.check(_lib.lib().zs_memcpy_dtoh(self.ctx.handle, out.ctypes.data,
+                                             ctypes.c_void_p(self._ptr), out.nbytes))
+        return FieldArray(out, f)
 
     @property
     def dtype(self) -> Field:
         return self.field
 
     def data_ptr(self) -> int:
-        return int(self.tensor.data_ptr())
+        if self._ptr is None:
+            raise ValueError("device tables were freed")
+        return self._ptr
+
+    def free(self) -> None:
+        if self._ptr is not None and self._owned:
+            _lib.check(_lib.lib().zs_device_free(self.ctx.handle, ctypes.c_void_p(self._ptr)))
+        self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def tables_pointer(state) -> tuple[int, bool, Any, tuple[int, ...]]:
