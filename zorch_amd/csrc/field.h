@@ -109,9 +109,9 @@ ZS_HD bb4 bb4_sub(bb4 a, bb4 b) {
 ZS_HD bb4 bb4_scale(bb4 a, uint32_t s) {  // extension times base
   return bb4_make(bb_mul(a.c[0], s), bb_mul(a.c[1], s), bb_mul(a.c[2], s), bb_mul(a.c[3], s));
 }
-// Multiplication with the X^4 = 11 wrap folded into b's high limbs.
-ZS_HD bb4 bb4_mul(bb4 a, bb4 b) {
-  uint32_t b1w = bb_mul11(b.c[1]), b2w = bb_mul11(b.c[2]), b3w = bb_mul11(b.c[3]);
+// Multiplication with the X^4 = 11 wrap folded into b's high limbs:
+// b1w, b2w, b3w = 11 b1, 11 b2, 11 b3 (mod p).
+ZS_HD bb4 bb4_mul_w(bb4 a, bb4 b, uint32_t b1w, uint32_t b2w, uint32_t b3w) {
   uint64_t t0 = (uint64_t)a.c[0] * b.c[0];
   t0 += (uint64_t)a.c[1] * b3w;
   t0 += (uint64_t)a.c[2] * b2w;
@@ -130,6 +130,9 @@ ZS_HD bb4 bb4_mul(bb4 a, bb4 b) {
   t3 += (uint64_t)a.c[3] * b.c[0];
   return bb4_make(bb_redc(bb_pre(t0)), bb_redc(bb_pre(t1)), bb_redc(bb_pre(t2)),
                   bb_redc(bb_pre(t3)));
+}
+ZS_HD bb4 bb4_mul(bb4 a, bb4 b) {
+  return bb4_mul_w(a, b, bb_mul11(b.c[1]), bb_mul11(b.c[2]), bb_mul11(b.c[3]));
 }
 ZS_HD bool bb4_eq(bb4 a, bb4 b) {
   return a.c[0] == b.c[0] && a.c[1] == b.c[1] && a.c[2] == b.c[2] && a.c[3] == b.c[3];

@@ -146,7 +146,7 @@ class InstanceImpl final : public Instance {
       const size_t pairs = (size_t)1 << (n_ - r - 1);
       if (pairs <= tune_.tail_pairs) {
         hipEvent_t ev = begin(s);
-        tail_kernel<FT, FC, K, NPTS, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
+        tail_kernel<FT, FC, K, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
         ZS_HIP(hipGetLastError());
         if (ev) {
           uint64_t b = 0;
@@ -160,27 +160,22 @@ class InstanceImpl final : public Instance {
       uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
       if (grid > maxg) grid = maxg;
       if (grid < 1) grid = 1;
+      // A power-of-two grid keeps each thread's eq_r index fixed across its pairs.
+      while (grid & (grid - 1)) grid &= grid - 1;
       const uint32_t L = 1u << (n_ - r);
       hipEvent_t ev = begin(s);
       if (r == 0) {
-        pass_kernel<FT, FT, FC, K, NPTS, EQ, false>
+        pass_kernel<FT, FT, FC, K, EQ, false>
             <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r);
       } else if (r == 1) {
-        pass_kernel<FT, FT, FC, K, NPTS, EQ, true>
+        pass_kernel<FT, FT, FC, K, EQ, true>
             <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r);
       } else {
-        pass_kernel<FT, FC, FC, K, NPTS, EQ, true>
+        pass_kernel<FT, FC, FC, K, EQ, true>
             <<<grid, PASS_THREADS, 0, s>>>(args_, args_.work, args_.work_stride, L, r);
       }
       ZS_HIP(hipGetLastError());
       if (ev) mark(s, r, KREC_PASS, (uint32_t)grid, pass_bytes(r), ev);
-      const TT* last_src = (r == 0) ? args_.in : reinterpret_cast<const TT*>(args_.work);
-      const size_t last_stride = (r == 0) ? args_.in_stride : args_.work_stride;
-      ev = begin(s);
-      finish_kernel<FT, FC, K, NPTS, EQ><<<1, PASS_THREADS, 0, s>>>(
-          args_, (uint32_t)grid, r, last_src, last_stride, r == 0 ? 1 : 0);
-      ZS_HIP(hipGetLastError());
-      if (ev) mark(s, r, KREC_FINISH, 1, 0, ev);
     }
     return ZS_OK;
   }

@@ -5,16 +5,19 @@
 // one streaming pass over HBM that
 //   1. folds the previous round's tables at the previous challenge,
 //      T'[x] = T[x] + r (T[x + L] - T[x])         (domain.py:226-233, MSB bind),
-//   2. evaluates this round's polynomial on the natural domain {0..D},
-//      s(t) = sum_j w(j) prod_k (lo_k[j] + t (hi_k[j] - lo_k[j]))
+//   2. evaluates this round's polynomial on the natural nodes t = 0..K,
+//      t(t) = sum_j w(j) prod_k (lo_k[j] + t (hi_k[j] - lo_k[j]))
 //                                               (domain.py:131-137,158-205),
 //      with w the factored eq weight eq_l[j_hi] eq_r[j_lo] in eq mode
-//      (eq_poly.py:63-97,144-187), never materialised,
-// then a one-block finisher reduces the per-block partials, runs the SHA-256
-// Fiat-Shamir hop on the device (sha256_fs_hop), reduces the claim by Lagrange
-// interpolation (reduce.py:27-52, poly/univariate.py:36-65) and binds the
-// challenge (round.py:41-49). The challenge stays in device memory, so a whole
-// stage is a chain of launches on one stream with no host round trip.
+//      (eq_poly.py:63-97,144-187), never materialised.
+// The last block to finish (a ticket counter) reduces the per-block partials,
+// extends t to node K+1 in eq mode (a degree-K polynomial is fixed by K+1
+// values, so t(K+1) = sum_i (-1)^(K-i) C(K+1, i) t(i) exactly), forms the
+// message s(t) = l(t) t(t) (eq_poly.py:100-111), runs the SHA-256 Fiat-Shamir
+// hop on one wavefront (hop.h), reduces the claim by Lagrange interpolation
+// (reduce.py:27-52, poly/univariate.py:36-65) and binds the challenge
+// (round.py:41-49). The challenge stays in device memory, so a whole stage is
+// a chain of launches on one stream with no host round trip.
 //
 // Layout: tables are table-major (table q at q*stride), element-contiguous;
 // BabyBear^4 elements are 16 B and move as one dwordx4. Pair j of a round with
@@ -23,6 +26,12 @@
 // folded entries j and j + L/2 — only positions thread j itself read, so the
 // fold runs in place after the first round. Consecutive lanes touch
 // consecutive addresses in all four streams.
+//
+// Eq weights: with a power-of-two grid whose thread count is a multiple of
+// |eq_r|, every pair a thread visits has the same low index bits, so eq_r is
+// a per-thread constant applied once after the loop and each pair multiplies
+// by eq_l only. When the tables are already in the challenge field the weight
+// is folded into the first factor (w lo_0, w hi_0) before the node products.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,6 +39,7 @@
 #include <type_traits>
 
 #include "field.h"
+#include "hop.h"
 #include "sha256.h"
 
 namespace zs {
@@ -46,7 +56,7 @@ struct Ctl {
   Sha256State sha;
   uint32_t item_bytes;   // transcript word size
   uint32_t limbs;        // transcript words per challenge
-  uint32_t counter;
+  uint32_t counter;      // last-block ticket of the current pass
   uint32_t pad;
   typename FC::T claim;
   typename FC::T r;        // current challenge, canonical
@@ -73,6 +83,41 @@ struct StageArgs {
   const typename FC::T* eq_r_fam;  // suffix family of eq_point[n/2 : n]
 };
 
+// --------------------------------------------------- challenge multiplier
+// Multiplication by the round challenge r, with the X^4 = 11 wrap of r
+// precomputed once per thread for BabyBear^4.
+template <class FC>
+struct ChalMul {
+  typename FC::T r;
+  __device__ __forceinline__ void init(typename FC::T v) { r = v; }
+  __device__ __forceinline__ typename FC::T mul(typename FC::T x) const { return FC::mul(x, r); }
+};
+template <>
+struct ChalMul<BB4> {
+  bb4 r;
+  uint32_t w1, w2, w3;
+  __device__ __forceinline__ void init(bb4 v) {
+    r = v;
+    w1 = bb_mul11(v.c[1]);
+    w2 = bb_mul11(v.c[2]);
+    w3 = bb_mul11(v.c[3]);
+  }
+  __device__ __forceinline__ bb4 mul(bb4 x) const { return bb4_mul_w(x, r, w1, w2, w3); }
+};
+
+template <class FC, class FS>
+struct FoldMul {
+  static __device__ __forceinline__ typename FC::T go(const ChalMul<FC>& cm, typename FS::T x) {
+    return cm.mul(x);
+  }
+};
+template <>
+struct FoldMul<BB4, BB> {
+  static __device__ __forceinline__ bb4 go(const ChalMul<BB4>& cm, uint32_t x) {
+    return bb4_scale(cm.r, x);
+  }
+};
+
 // ------------------------------------------------------------ reductions
 template <class F>
 __device__ __forceinline__ typename F::T shfl_xor_el(typename F::T v, int m);
@@ -91,14 +136,13 @@ __device__ __forceinline__ uint64_t shfl_xor_el<GL>(uint64_t v, int m) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Block-wide field sum of NPTS accumulators; the result lands in out[] (LDS)
+// Block-wide field sum of N accumulators; the result lands in out[] (LDS)
 // and is valid for every thread after the trailing barrier.
-template <class F, int NPTS, int THREADS>
-__device__ void block_reduce(typename F::T (&acc)[NPTS], typename F::T* lds,
-                             typename F::T* out) {
+template <class F, int N, int THREADS>
+__device__ __forceinline__ void block_reduce(typename F::T (&acc)[N], typename F::T* lds, typename F::T* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int t = 0; t < NPTS; ++t) {
+  for (int t = 0; t < N; ++t) {
     typename F::T v = acc[t];
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v = F::add(v, shfl_xor_el<F>(v, m));
@@ -106,21 +150,42 @@ __device__ void block_reduce(typename F::T (&acc)[NPTS], typename F::T* lds,
   }
   if (lane == 0) {
 #pragma unroll
-    for (int t = 0; t < NPTS; ++t) lds[wave * NPTS + t] = acc[t];
+    for (int t = 0; t < N; ++t) lds[wave * N + t] = acc[t];
   }
   __syncthreads();
-  if (threadIdx.x < NPTS) {
+  if (threadIdx.x < N) {
     typename F::T s = F::zero();
-    for (int w = 0; w < THREADS / 64; ++w) s = F::add(s, lds[w * NPTS + threadIdx.x]);
+    for (int w = 0; w < THREADS / 64; ++w) s = F::add(s, lds[w * N + threadIdx.x]);
     out[threadIdx.x] = s;
   }
   __syncthreads();
 }
 
+// Coherent (device-scope) word access for the cross-block partials.
+template <class F>
+__device__ __forceinline__ void store_coherent(typename F::T* p, typename F::T v) {
+  constexpr int W = sizeof(typename F::T) / 4;
+  uint32_t w[W];
+  __builtin_memcpy(w, &v, sizeof(v));
+  uint32_t* d = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+  for (int i = 0; i < W; ++i) __hip_atomic_store(d + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class F>
+__device__ __forceinline__ typename F::T load_coherent(const typename F::T* p) {
+  constexpr int W = sizeof(typename F::T) / 4;
+  uint32_t w[W];
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __hip_atomic_load(s + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  typename F::T v;
+  __builtin_memcpy(&v, w, sizeof(v));
+  return v;
+}
+
 // ------------------------------------------------------------- eq weights
-// Round index i = round + 1 (reference numbering). Returns the weight tables
-// for this round: eq_wl (null when only one half weights) and eq_wr, with
-// nxr = |eq_wr| (eq_poly.py:144-187).
+// Round index i = round + 1 (reference numbering): both half tables while
+// i < n/2, then the right family alone, then weight one (eq_poly.py:144-187).
 __device__ __host__ inline size_t fam_offset(uint32_t member) {
   return ((size_t)2 << member) - 2;  // member m has 2^(m+1) entries
 }
@@ -143,21 +208,21 @@ __device__ inline void select_eq(uint32_t round, uint32_t n, const typename FC::
 }
 
 // ------------------------------------------------------------ pass body
-// Accumulate this thread's pairs j = start, start+stride, ... < L/2 of round
-// `round`. FOLD: source tables have length 2L and are folded at r into dst
-// first; else the source tables are this round's tables (length L).
-template <class FS, class FC, int K, int NPTS, bool EQ, bool FOLD>
+// Accumulate this thread's pairs j = start, start+stride, ... < L/2 into
+// acc[t] = t(t), t = 0..K. FOLD: source tables have length 2L and are folded
+// at r into dst first; else the source tables are this round's tables.
+template <class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void pass_accumulate(
-    const typename FS::T* src, size_t src_stride, typename FC::T* dst,
-    size_t dst_stride, uint32_t L, typename FC::T r, const typename FC::T* wl,
-    const typename FC::T* wr, uint32_t nxr_log, size_t start, size_t stride,
-    typename FC::T (&acc)[NPTS]) {
-  // Values are computed in FV: the source field before the first fold, the
-  // challenge field after.
+    const typename FS::T* src, size_t src_stride, typename FC::T* dst, size_t dst_stride,
+    uint32_t L, const ChalMul<FC>& cm, const typename FC::T* wl, const typename FC::T* wr,
+    uint32_t nxr_log, size_t start, size_t stride, typename FC::T (&acc)[K + 1]) {
   using FV = typename std::conditional<FOLD, FC, FS>::type;
   using V = typename FV::T;
+  using T = typename FC::T;
+  constexpr bool WEIGHT_FIRST = EQ && std::is_same<FV, FC>::value;
   const size_t half = L >> 1;
   const size_t nxr_mask = ((size_t)1 << nxr_log) - 1;
+  const bool factored = EQ && wl != nullptr && (stride & nxr_mask) == 0;
   for (size_t j = start; j < half; j += stride) {
     V lo[K], df[K];
 #pragma unroll
@@ -165,9 +230,9 @@ __device__ __forceinline__ void pass_accumulate(
       const typename FS::T* s = src + (size_t)q * src_stride;
       if constexpr (FOLD) {
         typename FS::T s0 = s[j], s1 = s[j + L], s2 = s[j + half], s3 = s[j + half + L];
-        V a = FC::add(Lift<FS, FC>::go(s0), MixMul<FC, FS>::go(r, FS::sub(s1, s0)));
-        V b = FC::add(Lift<FS, FC>::go(s2), MixMul<FC, FS>::go(r, FS::sub(s3, s2)));
-        typename FC::T* d = dst + (size_t)q * dst_stride;
+        V a = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, FS::sub(s1, s0)));
+        V b = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, FS::sub(s3, s2)));
+        T* d = dst + (size_t)q * dst_stride;
         d[j] = a;
         d[j + half] = b;
         lo[q] = a;
@@ -178,44 +243,64 @@ __device__ __forceinline__ void pass_accumulate(
         df[q] = FV::sub(b, a);
       }
     }
-    typename FC::T w;
     if constexpr (EQ) {
-      w = wl ? FC::mul(wl[j >> nxr_log], wr[j & nxr_mask]) : wr[j];
-    }
-#pragma unroll
-    for (int t = 0; t < NPTS; ++t) {
-      V prod = lo[0];
-#pragma unroll
-      for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-      if constexpr (EQ) {
-        acc[t] = FC::add(acc[t], MixMul<FC, FV>::go(w, prod));
+      T w;
+      if (wl) {
+        w = wl[j >> nxr_log];
+        if (!factored) w = FC::mul(w, wr[j & nxr_mask]);
       } else {
-        acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
+        w = wr[j];
       }
-      if (t + 1 < NPTS) {
+      if constexpr (WEIGHT_FIRST) {
+        V hi0 = FV::add(lo[0], df[0]);
+        lo[0] = FC::mul(w, lo[0]);
+        df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
 #pragma unroll
-        for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+        for (int t = 0; t <= K; ++t) {
+          V prod = lo[0];
+#pragma unroll
+          for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+          acc[t] = FC::add(acc[t], prod);
+          if (t < K) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t <= K; ++t) {
+          V prod = lo[0];
+#pragma unroll
+          for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+          acc[t] = FC::add(acc[t], MixMul<FC, FV>::go(w, prod));
+          if (t < K) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t <= K; ++t) {
+        V prod = lo[0];
+#pragma unroll
+        for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+        acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
+        if (t < K) {
+#pragma unroll
+          for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+        }
       }
     }
+  }
+  if (factored && start < half) {
+    const T wrc = wr[start & nxr_mask];
+#pragma unroll
+    for (int t = 0; t <= K; ++t) acc[t] = FC::mul(acc[t], wrc);
   }
 }
 
 // --------------------------------------------------------- element bytes
-template <class F>
-__device__ __forceinline__ void el_to_bytes(typename F::T v, uint8_t* out);
-template <>
-__device__ __forceinline__ void el_to_bytes<BB>(uint32_t v, uint8_t* out) {
-  for (int i = 0; i < 4; ++i) out[i] = (uint8_t)(v >> (8 * i));
-}
-template <>
-__device__ __forceinline__ void el_to_bytes<BB4>(bb4 v, uint8_t* out) {
-  for (int c = 0; c < 4; ++c)
-    for (int i = 0; i < 4; ++i) out[4 * c + i] = (uint8_t)(v.c[c] >> (8 * i));
-}
-template <>
-__device__ __forceinline__ void el_to_bytes<GL>(uint64_t v, uint8_t* out) {
-  for (int i = 0; i < 8; ++i) out[i] = (uint8_t)(v >> (8 * i));
-}
 // Raw challenge bytes -> canonical element (a squeezed word may be >= p; its
 // value is the residue, reference zorch/transcript.py:97-119).
 template <class F>
@@ -239,19 +324,45 @@ __device__ __forceinline__ uint64_t el_from_bytes<GL>(const uint8_t* b) {
   return gl_canon(w);
 }
 
-// ----------------------------------------------------------- finisher
-// Thread 0 only: message from the reduced sums, FS hop, claim reduction,
-// bind, eq-mass update, and (last round) the final fold.
-template <class FS, class FC, int K, int NPTS, bool EQ>
-__device__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
-                             const typename FC::T* sums, const typename FS::T* last_src,
-                             size_t last_src_stride, bool last_src_is_input,
-                             Sha256State* sh, uint8_t* bytes) {
+// LDS scratch of the finisher.
+template <class FC>
+struct FinishLds {
+  alignas(16) uint8_t pend[64];
+  alignas(16) uint8_t tail[128];
+  alignas(16) typename FC::T msg[MAXPTS];
+  alignas(16) uint8_t buf[MAXPTS * 16 + 32];
+  alignas(16) uint8_t out[64];
+};
+
+// Wave 0 only (all 64 lanes, uniform values): message from the node sums,
+// Fiat-Shamir hop, claim reduction, bind, eq-mass update and (last round)
+// the final fold.
+template <class FS, class FC, int K, bool EQ>
+__device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
+                             const typename FC::T* sums, const typename FC::T* last_lo_hi_src,
+                             const typename FS::T* last_in, size_t last_stride,
+                             bool last_src_is_input, FinishLds<FC>& sm) {
   using T = typename FC::T;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  constexpr int EB = sizeof(T);
+  const uint32_t lane = threadIdx.x & 63;
   Ctl<FC>* ctl = a.ctl;
-  T msg[NPTS];
+  T tv[NPTS];
+#pragma unroll
+  for (int t = 0; t <= K; ++t) tv[t] = sums[t];
   T w_i = FC::zero();
+  T msg[NPTS];
   if constexpr (EQ) {
+    // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i).
+    T ext = FC::zero();
+    int64_t binom = 1;  // C(K+1, i)
+#pragma unroll
+    for (int i = 0; i <= K; ++i) {
+      const int64_t c = ((K - i) & 1) ? -binom : binom;
+      ext = FC::add(ext, FC::mul(tv[i], FC::from_int(c)));
+      binom = binom * (K + 1 - i) / (i + 1);
+    }
+    tv[K + 1] = ext;
     w_i = a.eq_point[round];
     T eqp = ctl->eq_prev;
     T l1 = FC::mul(eqp, w_i);
@@ -260,159 +371,163 @@ __device__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
     T lt = l0;
 #pragma unroll
     for (int t = 0; t < NPTS; ++t) {
-      msg[t] = FC::mul(lt, sums[t]);
+      msg[t] = FC::mul(lt, tv[t]);
       lt = FC::add(lt, ld);
     }
   } else {
 #pragma unroll
-    for (int t = 0; t < NPTS; ++t) msg[t] = sums[t];
+    for (int t = 0; t < NPTS; ++t) msg[t] = tv[t];
   }
-  constexpr int EB = FC::bytes;
+  if (lane == 0) {
 #pragma unroll
-  for (int t = 0; t < NPTS; ++t) {
-    a.msgs[(size_t)round * NPTS + t] = msg[t];
-    el_to_bytes<FC>(msg[t], bytes + t * EB);
+    for (int t = 0; t < NPTS; ++t) {
+      a.msgs[(size_t)round * NPTS + t] = msg[t];
+      sm.msg[t] = msg[t];
+    }
   }
-  *sh = ctl->sha;
-  uint8_t* raw = bytes + NPTS * EB;
-  sha256_fs_hop(*sh, bytes, (size_t)NPTS * EB, ctl->item_bytes, ctl->limbs, raw);
-  ctl->sha = *sh;
-  for (int i = 0; i < EB; ++i) a.point[(size_t)round * EB + i] = raw[i];
-  T r = el_from_bytes<FC>(raw);
-  // Lagrange basis at r over {0..NPTS-1}: prefix/suffix products of (r - j).
-  T diff[NPTS], pre[NPTS + 1], suf[NPTS + 1];
-#pragma unroll
-  for (int j = 0; j < NPTS; ++j) diff[j] = FC::sub(r, FC::from_int(j));
-  pre[0] = FC::one();
-  suf[NPTS] = FC::one();
-#pragma unroll
-  for (int j = 0; j < NPTS; ++j) pre[j + 1] = FC::mul(pre[j], diff[j]);
-#pragma unroll
-  for (int j = NPTS - 1; j >= 0; --j) suf[j] = FC::mul(suf[j + 1], diff[j]);
+  wave_sync();
+  WaveSha ws;
+  ws_load(ws, ctl->sha, sm.pend);
+  ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
+            ctl->item_bytes, ctl->limbs, sm.tail, sm.out);
+  ws_store(ws, ctl->sha, sm.pend);
+  if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
+  const T r = el_from_bytes<FC>(sm.out);
+  // Lagrange basis at r over {0..NPTS-1}, one node at a time (few live values):
+  // L_t(r) = prod_{j != t} (r - j) / prod_{j != t} (t - j).
   T claim = FC::zero();
-#pragma unroll
-  for (int t = 0; t < NPTS; ++t)
-    claim = FC::add(claim, FC::mul(msg[t], FC::mul(FC::mul(pre[t], suf[t + 1]), ctl->lag_inv[t])));
-  ctl->claim = claim;
-  ctl->r = r;
+#pragma unroll 1
+  for (int t = 0; t < NPTS; ++t) {
+    T num = ctl->lag_inv[t];
+#pragma unroll 1
+    for (int j = 0; j < NPTS; ++j)
+      if (j != t) num = FC::mul(num, FC::sub(r, FC::from_int(j)));
+    claim = FC::add(claim, FC::mul(sm.msg[t], num));
+  }
+  T eq_next = FC::zero();
   if constexpr (EQ) {
     T one = FC::one();
-    T ef = FC::add(FC::mul(r, w_i), FC::mul(FC::sub(one, r), FC::sub(one, w_i)));
-    ctl->eq_prev = FC::mul(ctl->eq_prev, ef);
+    const T wi = a.eq_point[round];
+    T ef = FC::add(FC::mul(r, wi), FC::mul(FC::sub(one, r), FC::sub(one, wi)));
+    eq_next = FC::mul(ctl->eq_prev, ef);
   }
-  if (round + 1 == a.n && a.final_out) {
-    for (uint32_t q = 0; q < a.k; ++q) {
-      T lo, hi;
-      if (last_src_is_input) {
-        lo = Lift<FS, FC>::go(last_src[(size_t)q * last_src_stride]);
-        hi = Lift<FS, FC>::go(last_src[(size_t)q * last_src_stride + 1]);
-      } else {
-        const T* s = reinterpret_cast<const T*>(last_src);
-        lo = s[(size_t)q * last_src_stride];
-        hi = s[(size_t)q * last_src_stride + 1];
-      }
-      a.final_out[q] = FC::add(lo, FC::mul(r, FC::sub(hi, lo)));
+  if (lane == 0) {
+    ctl->claim = claim;
+    ctl->r = r;
+    if constexpr (EQ) ctl->eq_prev = eq_next;
+  }
+  if (round + 1 == a.n && a.final_out && lane < a.k) {
+    const uint32_t q = lane;
+    T lo, hi;
+    if (last_src_is_input) {
+      lo = Lift<FS, FC>::go(last_in[(size_t)q * last_stride]);
+      hi = Lift<FS, FC>::go(last_in[(size_t)q * last_stride + 1]);
+    } else {
+      lo = last_lo_hi_src[(size_t)q * last_stride];
+      hi = last_lo_hi_src[(size_t)q * last_stride + 1];
     }
-    if constexpr (EQ) a.final_out[a.k] = ctl->eq_prev;
+    a.final_out[q] = FC::add(lo, FC::mul(r, FC::sub(hi, lo)));
+    if (EQ && q == 0) a.final_out[a.k] = eq_next;
   }
 }
 
 // ------------------------------------------------------------- kernels
-// Multi-block pass of one round: partial sums per block.
-template <class FT, class FS, class FC, int K, int NPTS, bool EQ, bool FOLD>
+// Multi-block pass of one round. Every block writes its partial node sums;
+// the last block to take a ticket reduces them and finishes the round.
+template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
 __global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
                                                             const typename FS::T* src,
                                                             size_t src_stride, uint32_t L,
                                                             uint32_t round) {
   using T = typename FC::T;
-  __shared__ T lds[(PASS_THREADS / 64) * NPTS];
-  __shared__ T out[NPTS];
-  T acc[NPTS];
+  constexpr int NA = K + 1;
+  __shared__ T lds[(PASS_THREADS / 64) * NA];
+  __shared__ T out[NA];
+  __shared__ FinishLds<FC> sm;
+  __shared__ uint32_t is_last;
+  T acc[NA];
 #pragma unroll
-  for (int t = 0; t < NPTS; ++t) acc[t] = FC::zero();
+  for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
   const T* wl = nullptr;
   const T* wr = nullptr;
   uint32_t nxr_log = 0;
   if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
-  T r = FC::zero();
-  if constexpr (FOLD) r = a.ctl->r;
+  ChalMul<FC> cm;
+  if constexpr (FOLD) cm.init(a.ctl->r);
   const size_t start = (size_t)blockIdx.x * PASS_THREADS + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * PASS_THREADS;
-  pass_accumulate<FS, FC, K, NPTS, EQ, FOLD>(src, src_stride, a.work, a.work_stride, L, r, wl,
-                                             wr, nxr_log, start, stride, acc);
-  block_reduce<FC, NPTS, PASS_THREADS>(acc, lds, out);
-  if (threadIdx.x < NPTS) a.partials[(size_t)blockIdx.x * NPTS + threadIdx.x] = out[threadIdx.x];
-}
-
-// One-block finisher of a multi-block pass.
-template <class FS, class FC, int K, int NPTS, bool EQ>
-__global__ __launch_bounds__(PASS_THREADS) void finish_kernel(StageArgs<FS, FC> a, uint32_t nblocks,
-                                                              uint32_t round,
-                                                              const typename FS::T* last_src,
-                                                              size_t last_src_stride,
-                                                              int last_src_is_input) {
-  using T = typename FC::T;
-  __shared__ T lds[(PASS_THREADS / 64) * NPTS];
-  __shared__ T out[NPTS];
-  __shared__ Sha256State sh;
-  __shared__ uint8_t bytes[MAXPTS * 16 + 64];
-  T acc[NPTS];
-#pragma unroll
-  for (int t = 0; t < NPTS; ++t) acc[t] = FC::zero();
-  for (uint32_t b = threadIdx.x; b < nblocks; b += PASS_THREADS) {
-#pragma unroll
-    for (int t = 0; t < NPTS; ++t) acc[t] = FC::add(acc[t], a.partials[(size_t)b * NPTS + t]);
+  pass_accumulate<FS, FC, K, EQ, FOLD>(src, src_stride, a.work, a.work_stride, L, cm, wl, wr,
+                                       nxr_log, start, stride, acc);
+  block_reduce<FC, NA, PASS_THREADS>(acc, lds, out);
+  if (threadIdx.x < NA) store_coherent<FC>(a.partials + (size_t)blockIdx.x * NA + threadIdx.x,
+                                           out[threadIdx.x]);
+  // Partial stores complete before the ticket is taken.
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t prev =
+        __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1) ? 1u : 0u;
   }
-  block_reduce<FC, NPTS, PASS_THREADS>(acc, lds, out);
+  __syncthreads();
+  if (!is_last) return;
+  T red[NA];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) red[t] = FC::zero();
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += PASS_THREADS) {
+#pragma unroll
+    for (int t = 0; t < NA; ++t)
+      red[t] = FC::add(red[t], load_coherent<FC>(a.partials + (size_t)b * NA + t));
+  }
+  block_reduce<FC, NA, PASS_THREADS>(red, lds, out);
+  if (threadIdx.x < 64) {
+    const T* last_work = a.work;
+    finish_round<FT, FC, K, EQ>(a, round, out, last_work, a.in,
+                                round == 0 ? a.in_stride : a.work_stride, round == 0, sm);
+  }
   if (threadIdx.x == 0)
-    finish_round<FS, FC, K, NPTS, EQ>(a, round, out, last_src, last_src_stride,
-                                      last_src_is_input != 0, &sh, bytes);
+    __hip_atomic_store(&a.ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Single-block kernel for the small tail rounds [first, n): each round is a
-// block-stride pass, a block reduction and the finisher, separated by
+// Single-block kernel for the small tail rounds [first, last): each round is
+// a block-stride pass, a block reduction and the finisher, separated by
 // barriers. Global writes of one round are visible to the block in the next.
-template <class FS, class FC, int K, int NPTS, bool EQ>
+template <class FS, class FC, int K, bool EQ>
 __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a, uint32_t first,
                                                             uint32_t last) {
   using T = typename FC::T;
-  __shared__ T lds[(TAIL_THREADS / 64) * NPTS];
-  __shared__ T out[NPTS];
-  __shared__ Sha256State sh;
-  __shared__ uint8_t bytes[MAXPTS * 16 + 64];
+  constexpr int NA = K + 1;
+  __shared__ T lds[(TAIL_THREADS / 64) * NA];
+  __shared__ T out[NA];
+  __shared__ FinishLds<FC> sm;
+  __shared__ T r_sh;
   for (uint32_t round = first; round < last; ++round) {
     const uint32_t L = 1u << (a.n - round);
-    T acc[NPTS];
+    T acc[NA];
 #pragma unroll
-    for (int t = 0; t < NPTS; ++t) acc[t] = FC::zero();
+    for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
     const T* wl = nullptr;
     const T* wr = nullptr;
     uint32_t nxr_log = 0;
     if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
-    const T r = a.ctl->r;
+    ChalMul<FC> cm;
+    if (round > 0) cm.init(round == first ? a.ctl->r : r_sh);
     if (round == 0) {
-      pass_accumulate<FS, FC, K, NPTS, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L,
-                                                  r, wl, wr, nxr_log, threadIdx.x,
-                                                  TAIL_THREADS, acc);
+      pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
+                                            wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
     } else if (round == 1) {
-      pass_accumulate<FS, FC, K, NPTS, EQ, true>(a.in, a.in_stride, a.work, a.work_stride, L,
-                                                 r, wl, wr, nxr_log, threadIdx.x, TAIL_THREADS,
-                                                 acc);
+      pass_accumulate<FS, FC, K, EQ, true>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
+                                           wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
     } else {
-      pass_accumulate<FC, FC, K, NPTS, EQ, true>(a.work, a.work_stride, a.work, a.work_stride,
-                                                 L, r, wl, wr, nxr_log, threadIdx.x,
-                                                 TAIL_THREADS, acc);
+      pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride, L, cm,
+                                           wl, wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
     }
-    block_reduce<FC, NPTS, TAIL_THREADS>(acc, lds, out);
-    if (threadIdx.x == 0) {
-      if (round == 0)
-        finish_round<FS, FC, K, NPTS, EQ>(a, round, out, a.in, a.in_stride, true, &sh, bytes);
-      else
-        finish_round<FS, FC, K, NPTS, EQ>(a, round, out,
-                                          reinterpret_cast<const typename FS::T*>(a.work),
-                                          a.work_stride, false, &sh, bytes);
+    block_reduce<FC, NA, TAIL_THREADS>(acc, lds, out);
+    if (threadIdx.x < 64) {
+      finish_round<FS, FC, K, EQ>(a, round, out, a.work, a.in,
+                                  round == 0 ? a.in_stride : a.work_stride, round == 0, sm);
+      if (threadIdx.x == 0) r_sh = a.ctl->r;
     }
-    __threadfence_block();
     __syncthreads();
   }
 }

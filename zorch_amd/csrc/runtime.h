@@ -50,7 +50,7 @@ class DevicePool {
 struct TuneParams {
   uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
   uint32_t max_grid = 2048;     // blocks of a multi-block pass (<= MAXG)
-  uint32_t pairs_per_thread = 4;  // target pairs per thread of a multi-block pass
+  uint32_t pairs_per_thread = 1;  // target pairs per thread of a multi-block pass
 };
 
 // Kernel timing records (enabled by zs_ctx_profile).

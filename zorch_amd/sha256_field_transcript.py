@@ -158,7 +158,8 @@ class Sha256FieldTranscript:
 
     def __eq__(self, other) -> bool:
         return (isinstance(other, Sha256FieldTranscript)
-                and bytes(self.state) == bytes(other.state))
+                and self.stream_state() == other.stream_state()
+                and self.item_bytes == other.item_bytes)
 
     def __hash__(self) -> int:
-        return hash(bytes(self.state))
+        return hash(self.stream_state())
