@@ -76,6 +76,32 @@ void DevicePool::release(void* p) {
   free_.emplace(it->second, p);
 }
 
+HostPool::~HostPool() {
+  for (auto& kv : sizes_) (void)hipHostFree(kv.first);
+}
+
+int HostPool::alloc(size_t bytes, void** out) {
+  bytes = (bytes + 4095) & ~(size_t)4095;
+  auto it = free_.find(bytes);
+  if (it != free_.end()) {
+    *out = it->second;
+    free_.erase(it);
+    return ZS_OK;
+  }
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return hip_fail(e, "hipHostMalloc");
+  sizes_[p] = bytes;
+  *out = p;
+  return ZS_OK;
+}
+
+void HostPool::release(void* p) {
+  auto it = sizes_.find(p);
+  if (it == sizes_.end()) return;
+  free_.emplace(it->second, p);
+}
+
 // ---------------------------------------------------------------- helpers
 uint32_t field_bytes(uint32_t field) {
   switch (field) {
@@ -206,8 +232,10 @@ int zs_ctx_create(int device, uint64_t budget, zs_ctx** out) {
     budget = (uint64_t)(free_b * 0.9);
   }
   ctx->pool = new DevicePool(budget);
+  ctx->host_pool = new HostPool();
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
+    delete ctx->host_pool;
     delete ctx->pool;
     delete ctx;
     return hip_fail(e, "hipStreamCreate");
@@ -225,6 +253,7 @@ void zs_ctx_destroy(zs_ctx* ctx) {
     (void)hipEventDestroy(r.stop);
   }
   for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  delete ctx->host_pool;
   delete ctx->pool;
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

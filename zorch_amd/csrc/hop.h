@@ -8,6 +8,11 @@
 // chaining value, pending length and total length are held uniformly in every
 // lane's registers; only the 64-byte pending block lives in LDS.
 //
+// A hop is a small schedule of compressions (absorb the framed message, one
+// or two tail blocks per squeeze counter, absorb the squeezed bytes). They all
+// go through one call site of the fully unrolled compression, so the 64-round
+// body is fetched into the instruction cache once per hop and then reused.
+//
 // Byte stream identical to sha256_fs_hop (sha256.h): observe(msg) under slice
 // framing, sample(limbs) (reference zorch/sha256_field_transcript.py:100-114,
 // 189-196, 251-259; zorch/challenge.py:97-106).
@@ -30,33 +35,6 @@ struct WaveSha {
   uint32_t plen;
   uint64_t total;
 };
-
-struct ShaH {
-  uint32_t v[8];
-};
-struct ShaW {
-  uint32_t v[16];
-};
-
-// One compression, kept out of line so each kernel carries a single copy of
-// the fully unrolled 64-round body; state and words travel in registers.
-__device__ __noinline__ ShaH ws_compress(ShaH h, ShaW w) {
-  sha256_compress_words(h.v, w.v);
-  return h;
-}
-
-__device__ __forceinline__ void ws_compress_lds(uint32_t h[8], const uint8_t* block) {
-  ShaW w;
-  const uint32_t* b32 = reinterpret_cast<const uint32_t*>(block);
-#pragma unroll
-  for (int i = 0; i < 16; ++i) w.v[i] = __builtin_bswap32(b32[i]);
-  ShaH s;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) s.v[i] = h[i];
-  s = ws_compress(s, w);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) h[i] = s.v[i];
-}
 
 // Load the wave state from a Sha256State (global or LDS) into registers and
 // the pending bytes into the LDS block `pend` (64 B, 4-byte aligned).
@@ -81,29 +59,10 @@ __device__ __forceinline__ void ws_store(const WaveSha& s, Sha256State& st, cons
   }
 }
 
-// Absorb `len` bytes of `src` (LDS).
-__device__ __forceinline__ void ws_absorb(WaveSha& s, uint8_t* pend, const uint8_t* src, uint32_t len) {
-  const uint32_t lane = threadIdx.x & 63;
-  uint32_t pos = 0;
-  while (pos < len) {
-    const uint32_t room = 64u - s.plen;
-    const uint32_t take = (len - pos) < room ? (len - pos) : room;
-    if (lane < take) pend[s.plen + lane] = src[pos + lane];
-    wave_sync();
-    s.plen += take;
-    s.total += take;
-    pos += take;
-    if (s.plen == 64u) {
-      ws_compress_lds(s.h, pend);
-      s.plen = 0;
-      wave_sync();
-    }
-  }
-}
-
-// Digest words of SHA256(stream || le64(ctr)); `tail` is 128 B of LDS.
-__device__ __forceinline__ void ws_digest_ctr(const WaveSha& s, const uint8_t* pend, uint64_t ctr, uint8_t* tail,
-                              uint32_t out[8]) {
+// Fill the padded tail of SHA256(stream || le64(ctr)) into `tail` (128 B of
+// LDS), one byte per lane; returns the number of tail blocks (1 or 2).
+__device__ __forceinline__ uint32_t ws_build_tail(const WaveSha& s, const uint8_t* pend,
+                                                  uint64_t ctr, uint8_t* tail) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t plen = s.plen, mlen = plen + 8u;
   const uint32_t nblk = (mlen + 9u <= 64u) ? 1u : 2u;
@@ -119,29 +78,19 @@ __device__ __forceinline__ void ws_digest_ctr(const WaveSha& s, const uint8_t* p
     tail[pos] = (uint8_t)b;
   }
   wave_sync();
-#pragma unroll
-  for (int i = 0; i < 8; ++i) out[i] = s.h[i];
-  ws_compress_lds(out, tail);
-  if (nblk == 2u) ws_compress_lds(out, tail + 64);
-  wave_sync();
-}
-
-__device__ __forceinline__ uint32_t ws_pick(const uint32_t w[8], uint32_t i) {
-  uint32_t r = w[0];
-#pragma unroll
-  for (uint32_t k = 1; k < 8; ++k) r = (i == k) ? w[k] : r;
-  return r;
+  return nblk;
 }
 
 // observe(msg) + sample(limbs): `msg` (LDS, msg_len bytes) is framed into
-// `buf` (LDS, >= msg_len + 20 bytes); the squeezed bytes land in `out` (LDS).
-__device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* buf, const uint8_t* msg,
-                          uint32_t msg_len, uint32_t item, uint32_t limbs, uint8_t* tail,
-                          uint8_t* out) {
+// `buf` (LDS, >= msg_len + 20 bytes); the squeezed bytes land in `out` (LDS,
+// 32-byte granules), and `tail` (LDS, 128 B) holds squeeze tail blocks.
+__device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* buf,
+                                          const uint8_t* msg, uint32_t msg_len, uint32_t item,
+                                          uint32_t limbs, uint8_t* tail, uint8_t* out) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t count = msg_len / item;
-  const uint32_t total = msg_len + 20u;
-  for (uint32_t pos = lane; pos < total; pos += 64u) {
+  const uint32_t framed = msg_len + 20u;
+  for (uint32_t pos = lane; pos < framed; pos += 64u) {
     uint32_t b;
     if (pos == 0) b = OP_OBSERVE;
     else if (pos == 1) b = KIND_SLICE;
@@ -153,23 +102,81 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
     buf[pos] = (uint8_t)b;
   }
   wave_sync();
-  ws_absorb(s, pend, buf, total);
   const uint32_t nbytes = item * limbs;
-  for (uint32_t ctr = 0; ctr * 32u < nbytes; ++ctr) {
-    uint32_t dg[8];
-    ws_digest_ctr(s, pend, ctr, tail, dg);
-    const uint32_t take = (nbytes - ctr * 32u) < 32u ? (nbytes - ctr * 32u) : 32u;
-    // Digest words to LDS big-endian (static indices), then one byte per lane.
+  // phase 0: absorb buf; 1: squeeze tail blocks; 2: absorb out; 3: done.
+  uint32_t phase = 0;
+  const uint8_t* src = buf;
+  uint32_t src_len = framed, src_pos = 0;
+  uint32_t ctr = 0, dblk = 0, dnblk = 0, got = 0;
+  uint32_t dg[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) dg[i] = 0;
+  while (phase != 3u) {
+    const uint8_t* blk;
+    bool squeeze;
+    if (phase != 1u) {
+      const uint32_t room = 64u - s.plen;
+      const uint32_t take = (src_len - src_pos) < room ? (src_len - src_pos) : room;
+      if (lane < take) pend[s.plen + lane] = src[src_pos + lane];
+      wave_sync();
+      s.plen += take;
+      s.total += take;
+      src_pos += take;
+      if (s.plen < 64u) {  // source exhausted, block not full
+        if (phase == 0u) {
+          phase = 1u;
+          dblk = 0;
+          continue;
+        }
+        phase = 3u;
+        break;
+      }
+      blk = pend;
+      squeeze = false;
+    } else {
+      if (dblk == 0) dnblk = ws_build_tail(s, pend, ctr, tail);
+      blk = tail + 64u * dblk;
+      squeeze = true;
+    }
+    // The one compression site.
+    uint32_t w[16], hh[8];
+    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(blk);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) w[i] = __builtin_bswap32(b32[i]);
+    const bool from_dg = squeeze && dblk > 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hh[i] = from_dg ? dg[i] : s.h[i];
+    sha256_compress_words(hh, w);
+    wave_sync();
+    if (!squeeze) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s.h[i] = hh[i];
+      s.plen = 0;
+      continue;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dg[i] = hh[i];
+    if (++dblk < dnblk) continue;
+    // Digest complete: its first bytes are squeeze output.
+    const uint32_t take = (nbytes - got) < 32u ? (nbytes - got) : 32u;
     uint32_t* tw = reinterpret_cast<uint32_t*>(tail);
     if (lane == 0) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) tw[i] = __builtin_bswap32(dg[i]);
     }
     wave_sync();
-    if (lane < take) out[ctr * 32u + lane] = tail[lane];
+    if (lane < take) out[got + lane] = tail[lane];
     wave_sync();
+    got += take;
+    ++ctr;
+    dblk = 0;
+    if (got >= nbytes) {
+      phase = 2u;
+      src = out;
+      src_len = nbytes;
+      src_pos = 0;
+    }
   }
-  ws_absorb(s, pend, out, nbytes);
 }
 
 }  // namespace zs

@@ -1,7 +1,15 @@
 // Typed proof instance: device buffers, uploads, the per-round launch
 // sequence and the downloads. Instantiated per field combination in
 // inst_*.hip (one translation unit each, compiled in parallel).
+//
+// Device block layout (one pool allocation, 256-byte aligned regions):
+//   io region:  control block | messages | raw point | final values | eq point
+//   partials | eq_l family | eq_r family | folded tables | staged input tables
+// The io region is mirrored by one pinned host block, so a proof costs one
+// host-to-device copy of the control block and eq point before its rounds and
+// one device-to-host copy of the outputs after them.
 #pragma once
+#include <stddef.h>
 #include <string.h>
 
 #include <vector>
@@ -38,17 +46,18 @@ class InstanceImpl final : public Instance {
     tune_ = ctx->tune;
     const size_t N = (size_t)1 << n_;
     const uint32_t l_half = n_ / 2;
-    // Carve one block.
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
-    const size_t o_ctl = take(sizeof(Ctl<FC>));
-    const size_t o_msgs = take((size_t)n_ * NPTS * sizeof(TC));
-    const size_t o_point = take((size_t)n_ * sizeof(TC));
-    const size_t o_final = take((size_t)(K + 1) * sizeof(TC));
-    const size_t o_part = take((size_t)MAXG * NPTS * sizeof(TC));
+    o_ctl_ = take(sizeof(Ctl<FC>));
+    o_msgs_ = take((size_t)n_ * NPTS * sizeof(TC));
+    o_point_ = take((size_t)n_ * sizeof(TC));
+    o_final_ = take((size_t)(K + 1) * sizeof(TC));
+    out_bytes_ = off;
     size_t o_eqp = 0, o_eql = 0, o_eqr = 0;
+    if (EQ) o_eqp = take((size_t)n_ * sizeof(TC));
+    io_bytes_ = off;
+    const size_t o_part = take((size_t)MAXG * NPTS * sizeof(TC));
     if (EQ) {
-      o_eqp = take((size_t)n_ * sizeof(TC));
       o_eql = take(((size_t)2 << l_half) * sizeof(TC));
       o_eqr = take(((size_t)2 << (n_ - l_half)) * sizeof(TC));
     }
@@ -60,9 +69,9 @@ class InstanceImpl final : public Instance {
     void* block = nullptr;
     int rc = ctx->pool->alloc(off, &block);
     if (rc) return rc;
-    bufs_.block = block;
-    bufs_.block_bytes = off;
+    block_ = block;
     device_bytes = off;
+    if ((rc = ctx->host_pool->alloc(io_bytes_, &host_))) return rc;
     uint8_t* b = (uint8_t*)block;
     args_.in = stage_in ? (const TT*)(b + o_in) : (const TT*)a.tables;
     args_.in_stride = N;
@@ -70,11 +79,11 @@ class InstanceImpl final : public Instance {
     args_.work_stride = work_stride;
     args_.n = n_;
     args_.k = K;
-    args_.ctl = (Ctl<FC>*)(b + o_ctl);
+    args_.ctl = (Ctl<FC>*)(b + o_ctl_);
     args_.partials = (TC*)(b + o_part);
-    args_.msgs = (TC*)(b + o_msgs);
-    args_.point = b + o_point;
-    args_.final_out = (TC*)(b + o_final);
+    args_.msgs = (TC*)(b + o_msgs_);
+    args_.point = b + o_point_;
+    args_.final_out = (TC*)(b + o_final_);
     args_.eq_point = EQ ? (const TC*)(b + o_eqp) : nullptr;
     args_.eq_l_fam = EQ ? (const TC*)(b + o_eql) : nullptr;
     args_.eq_r_fam = EQ ? (const TC*)(b + o_eqr) : nullptr;
@@ -83,38 +92,42 @@ class InstanceImpl final : public Instance {
       ZS_HIP(hipMemcpyAsync((void*)args_.in, a.tables, (size_t)K * N * sizeof(TT),
                             hipMemcpyHostToDevice, s));
     }
-    // Control block.
-    memset(&ctl_h_, 0, sizeof(ctl_h_));
-    memcpy(&ctl_h_.sha, &t.sha, sizeof(Sha256State));
-    ctl_h_.item_bytes = t.item_bytes;
-    ctl_h_.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
-    ctl_h_.claim = load_host_el<FC>(a.claim, 0);
-    ctl_h_.r = FC::zero();
-    ctl_h_.eq_prev = FC::one();
-    ctl_h_.one = FC::one();
+    // Control block and eq point, staged in pinned memory, one copy.
+    uint8_t* h = (uint8_t*)host_;
+    memset(h, 0, io_bytes_);
+    Ctl<FC>& c = *(Ctl<FC>*)(h + o_ctl_);
+    memcpy(&c.sha, &t.sha, sizeof(Sha256State));
+    c.item_bytes = t.item_bytes;
+    c.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
+    c.counter = 0;
+    c.claim = load_host_el<FC>(a.claim, 0);
+    c.r = FC::zero();
+    c.eq_prev = FC::one();
+    c.one = FC::one();
     for (int tt = 0; tt < NPTS; ++tt) {
       int64_t den = 1;
       for (int j = 0; j < NPTS; ++j)
         if (j != tt) den *= (int64_t)(tt - j);
-      ctl_h_.lag_inv[tt] = FC::inv(FC::from_int(den));
+      c.lag_inv[tt] = FC::inv(FC::from_int(den));
+      c.nodes[tt] = FC::from_int(tt);
     }
-    ZS_HIP(hipMemcpyAsync(args_.ctl, &ctl_h_, sizeof(ctl_h_), hipMemcpyHostToDevice, s));
+    int64_t binom = 1;  // C(K+1, i)
+    for (int i = 0; i <= K; ++i) {
+      c.ext_coef[i] = FC::from_int(((K - i) & 1) ? -binom : binom);
+      binom = binom * (K + 1 - i) / (i + 1);
+    }
     if (EQ) {
-      eq_h_.resize(n_);
-      for (uint32_t i = 0; i < n_; ++i) eq_h_[i] = load_host_el<FC>(a.eq_point, i);
-      ZS_HIP(hipMemcpyAsync((void*)args_.eq_point, eq_h_.data(), n_ * sizeof(TC),
-                            hipMemcpyHostToDevice, s));
-      if (l_half > 0)
-        eq_family_kernel<FC><<<1, 256, 0, s>>>(args_.eq_point, l_half, (TC*)args_.eq_l_fam);
-      if (n_ - l_half > 0)
-        eq_family_kernel<FC><<<1, 256, 0, s>>>(args_.eq_point + l_half, n_ - l_half,
-                                               (TC*)args_.eq_r_fam);
+      TC* ep = (TC*)(h + o_eqp);
+      for (uint32_t i = 0; i < n_; ++i) ep[i] = load_host_el<FC>(a.eq_point, i);
+    }
+    ZS_HIP(hipMemcpyAsync(b, h, io_bytes_, hipMemcpyHostToDevice, s));
+    if (EQ) {
+      const size_t entries = fam_offset(l_half) + fam_offset(n_ - l_half);
+      const size_t grid = (entries + 255) / 256;
+      eq_family_kernel<FC><<<grid, 256, 0, s>>>(args_.eq_point, l_half, (TC*)args_.eq_l_fam,
+                                                n_ - l_half, (TC*)args_.eq_r_fam);
       ZS_HIP(hipGetLastError());
     }
-    // Host staging for outputs.
-    msgs_h_.resize((size_t)n_ * NPTS);
-    point_h_.resize((size_t)n_ * sizeof(TC));
-    final_h_.resize(K + 1);
     return ZS_OK;
   }
 
@@ -181,51 +194,42 @@ class InstanceImpl final : public Instance {
   }
 
   int download(hipStream_t s) override {
-    ZS_HIP(hipMemcpyAsync(msgs_h_.data(), args_.msgs, msgs_h_.size() * sizeof(TC),
-                          hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipMemcpyAsync(point_h_.data(), args_.point, point_h_.size(),
-                          hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipMemcpyAsync(final_h_.data(), args_.final_out, final_h_.size() * sizeof(TC),
-                          hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipMemcpyAsync(&ctl_h_, args_.ctl, sizeof(ctl_h_), hipMemcpyDeviceToHost, s));
+    ZS_HIP(hipMemcpyAsync(host_, block_, out_bytes_, hipMemcpyDeviceToHost, s));
     return ZS_OK;
   }
 
   int collect(zs_proof* out, zs_transcript* t) override {
-    if (out->msgs) memcpy(out->msgs, msgs_h_.data(), msgs_h_.size() * sizeof(TC));
-    if (out->point) memcpy(out->point, point_h_.data(), point_h_.size());
-    if (out->value) memcpy(out->value, &ctl_h_.claim, sizeof(TC));
-    if (out->final_tables) memcpy(out->final_tables, final_h_.data(), (K + (EQ ? 1 : 0)) * sizeof(TC));
-    if (t) memcpy(&t->sha, &ctl_h_.sha, sizeof(Sha256State));
+    const uint8_t* h = (const uint8_t*)host_;
+    const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
+    if (out->msgs) memcpy(out->msgs, h + o_msgs_, (size_t)n_ * NPTS * sizeof(TC));
+    if (out->point) memcpy(out->point, h + o_point_, (size_t)n_ * sizeof(TC));
+    if (out->value) memcpy(out->value, &c.claim, sizeof(TC));
+    if (out->final_tables) memcpy(out->final_tables, h + o_final_, (K + (EQ ? 1 : 0)) * sizeof(TC));
+    if (t) memcpy(&t->sha, &c.sha, sizeof(Sha256State));
     return ZS_OK;
   }
 
   int upload_transcript(const zs_transcript& t, hipStream_t s) override {
     // The transcript prefix of the control block: state, word size, limbs.
-    memcpy(&ctl_h_.sha, &t.sha, sizeof(Sha256State));
-    ctl_h_.item_bytes = t.item_bytes;
-    ctl_h_.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
+    Ctl<FC>& c = *(Ctl<FC>*)((uint8_t*)host_ + o_ctl_);
+    memcpy(&c.sha, &t.sha, sizeof(Sha256State));
+    c.item_bytes = t.item_bytes;
+    c.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
     static_assert(offsetof(Ctl<FC>, limbs) + sizeof(uint32_t) == offsetof(Ctl<FC>, counter),
                   "control block prefix");
-    ZS_HIP(hipMemcpyAsync(args_.ctl, &ctl_h_, offsetof(Ctl<FC>, counter), hipMemcpyHostToDevice, s));
+    ZS_HIP(hipMemcpyAsync(args_.ctl, &c, offsetof(Ctl<FC>, counter), hipMemcpyHostToDevice, s));
     return ZS_OK;
   }
 
-  int download_round(uint32_t round, hipStream_t s) override {
-    ZS_HIP(hipMemcpyAsync(msgs_h_.data() + (size_t)round * NPTS, args_.msgs + (size_t)round * NPTS,
-                          NPTS * sizeof(TC), hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipMemcpyAsync(point_h_.data() + (size_t)round * sizeof(TC),
-                          args_.point + (size_t)round * sizeof(TC), sizeof(TC),
-                          hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipMemcpyAsync(&ctl_h_, args_.ctl, sizeof(ctl_h_), hipMemcpyDeviceToHost, s));
-    return ZS_OK;
-  }
+  int download_round(uint32_t, hipStream_t s) override { return download(s); }
 
   int collect_round(uint32_t round, void* msg, void* chal, void* claim, zs_transcript* t) override {
-    if (msg) memcpy(msg, msgs_h_.data() + (size_t)round * NPTS, NPTS * sizeof(TC));
-    if (chal) memcpy(chal, point_h_.data() + (size_t)round * sizeof(TC), sizeof(TC));
-    if (claim) memcpy(claim, &ctl_h_.claim, sizeof(TC));
-    if (t) memcpy(&t->sha, &ctl_h_.sha, sizeof(Sha256State));
+    const uint8_t* h = (const uint8_t*)host_;
+    const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
+    if (msg) memcpy(msg, h + o_msgs_ + (size_t)round * NPTS * sizeof(TC), NPTS * sizeof(TC));
+    if (chal) memcpy(chal, h + o_point_ + (size_t)round * sizeof(TC), sizeof(TC));
+    if (claim) memcpy(claim, &c.claim, sizeof(TC));
+    if (t) memcpy(&t->sha, &c.sha, sizeof(Sha256State));
     return ZS_OK;
   }
 
@@ -254,22 +258,20 @@ class InstanceImpl final : public Instance {
   }
 
   void free_buffers() override {
-    if (bufs_.block && ctx_) ctx_->pool->release(bufs_.block);
-    bufs_.block = nullptr;
+    if (block_ && ctx_) ctx_->pool->release(block_);
+    if (host_ && ctx_) ctx_->host_pool->release(host_);
+    block_ = nullptr;
+    host_ = nullptr;
   }
 
  private:
   zs_ctx* ctx_ = nullptr;
   uint32_t n_ = 0;
   TuneParams tune_;
-  InstanceBuffers bufs_;
+  void* block_ = nullptr;
+  void* host_ = nullptr;
+  size_t o_ctl_ = 0, o_msgs_ = 0, o_point_ = 0, o_final_ = 0, out_bytes_ = 0, io_bytes_ = 0;
   StageArgs<FT, FC> args_{};
-  Ctl<FC> ctl_h_;
-  Sha256State sha_h_;
-  std::vector<TC> eq_h_;
-  std::vector<TC> msgs_h_;
-  std::vector<uint8_t> point_h_;
-  std::vector<TC> final_h_;
 };
 
 template <class FT, class FC>

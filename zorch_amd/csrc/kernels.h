@@ -63,6 +63,8 @@ struct Ctl {
   typename FC::T eq_prev;  // eq mass of the bound prefix (eq mode)
   typename FC::T one;
   typename FC::T lag_inv[MAXPTS];  // 1 / prod_{j != t} (t - j) over {0..npts-1}
+  typename FC::T nodes[MAXPTS];    // t = 0..npts-1 as field elements
+  typename FC::T ext_coef[MAXPTS]; // (-1)^(K-i) C(K+1, i): t(K+1) from t(0..K)
 };
 
 template <class FT, class FC>
@@ -353,15 +355,11 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   T w_i = FC::zero();
   T msg[NPTS];
   if constexpr (EQ) {
-    // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i).
+    // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i)
+    // (coefficients precomputed in the control block).
     T ext = FC::zero();
-    int64_t binom = 1;  // C(K+1, i)
 #pragma unroll
-    for (int i = 0; i <= K; ++i) {
-      const int64_t c = ((K - i) & 1) ? -binom : binom;
-      ext = FC::add(ext, FC::mul(tv[i], FC::from_int(c)));
-      binom = binom * (K + 1 - i) / (i + 1);
-    }
+    for (int i = 0; i <= K; ++i) ext = FC::add(ext, FC::mul(tv[i], ctl->ext_coef[i]));
     tv[K + 1] = ext;
     w_i = a.eq_point[round];
     T eqp = ctl->eq_prev;
@@ -401,7 +399,7 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
     T num = ctl->lag_inv[t];
 #pragma unroll 1
     for (int j = 0; j < NPTS; ++j)
-      if (j != t) num = FC::mul(num, FC::sub(r, FC::from_int(j)));
+      if (j != t) num = FC::mul(num, FC::sub(r, ctl->nodes[j]));
     claim = FC::add(claim, FC::mul(sm.msg[t], num));
   }
   T eq_next = FC::zero();
@@ -532,26 +530,35 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
   }
 }
 
-// Suffix eq family of v[0:h] (compute_eq_evaluations, eq_poly.py:46-51):
-// member m = eq(v[h-1-m:], .) with the first coordinate on the MSB, stored at
-// fam_offset(m). One block; member m+1 = [member m * (1 - c), member m * c].
+// Suffix eq families of the two halves of the eq point (compute_eq_evaluations,
+// eq_poly.py:46-51): member m of the family of v[0:h] is eq(v[h-1-m:], .) over
+// {0,1}^(m+1) with v[h-1-m] on the MSB, stored at fam_offset(m). The reference
+// grows each member from the previous one ([prev (1 - c), prev c]); here every
+// entry is its product over coordinates, prod_i (bit_i ? c_i : 1 - c_i),
+// computed independently by one thread (the same field value, exactly).
 template <class FC>
-__global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v, uint32_t h,
-                                                        typename FC::T* fam) {
+__global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v, uint32_t h_l,
+                                                        typename FC::T* fam_l, uint32_t h_r,
+                                                        typename FC::T* fam_r) {
   using T = typename FC::T;
-  for (uint32_t m = 0; m < h; ++m) {
-    const T c = v[h - 1 - m];
-    T* dstm = fam + fam_offset(m);
-    const size_t len = (size_t)1 << m;  // previous member length
-    for (size_t x = threadIdx.x; x < len; x += blockDim.x) {
-      T prev = (m == 0) ? FC::one() : fam[fam_offset(m - 1) + x];
-      T hi = FC::mul(prev, c);
-      dstm[x] = FC::sub(prev, hi);
-      dstm[x + len] = hi;
-    }
-    __threadfence_block();
-    __syncthreads();
+  const size_t n_l = fam_offset(h_l), n_r = fam_offset(h_r);
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_l + n_r) return;
+  const bool left = g < n_l;
+  const size_t idx = left ? g : g - n_l;
+  const uint32_t h = left ? h_l : h_r;
+  const T* vv = left ? v : v + h_l;
+  T* fam = left ? fam_l : fam_r;
+  const uint32_t m = 31u - __builtin_clz((uint32_t)(idx + 2)) - 1u;  // fam_offset(m) <= idx
+  const size_t x = idx - fam_offset(m);
+  const T one = FC::one();
+  T acc = one;
+  for (uint32_t i = 0; i <= m; ++i) {
+    const T c = vv[h - 1 - m + i];
+    const bool bit = (x >> (m - i)) & 1u;
+    acc = FC::mul(acc, bit ? c : FC::sub(one, c));
   }
+  fam[idx] = acc;
 }
 
 }  // namespace zs

@@ -47,6 +47,19 @@ class DevicePool {
   std::map<void*, size_t> sizes_;
 };
 
+// Pinned host staging blocks, cached by size (uploads/downloads of the small
+// per-proof control and output regions go through one copy each).
+class HostPool {
+ public:
+  ~HostPool();
+  int alloc(size_t bytes, void** out);
+  void release(void* p);
+
+ private:
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> sizes_;
+};
+
 struct TuneParams {
   uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
   uint32_t max_grid = 2048;     // blocks of a multi-block pass (<= MAXG)
@@ -67,6 +80,7 @@ struct zs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   zs::DevicePool* pool = nullptr;
+  zs::HostPool* host_pool = nullptr;
   zs::TuneParams tune;
   bool profile = false;
   std::vector<zs::KernelRecord> records;
@@ -84,12 +98,6 @@ struct zs_ctx {
 };
 
 namespace zs {
-
-// Device buffers of one proof instance, carved from one pool block.
-struct InstanceBuffers {
-  void* block = nullptr;
-  size_t block_bytes = 0;
-};
 
 // One proof in flight (type-erased over field / factor count / mode).
 class Instance {
