@@ -126,6 +126,14 @@ typedef struct zs_kernel_record {
 } zs_kernel_record;
 int zs_ctx_profile(zs_ctx* ctx, int enable);
 int zs_ctx_profile_read(zs_ctx* ctx, zs_kernel_record* out, uint32_t cap, uint32_t* count);
+/* In-kernel phase trace (profiling aid): while enabled, kernels record
+ * (round << 8 | phase, device wall-clock ticks) pairs; read copies up to `cap`
+ * pairs, reports the wall-clock rate in kHz and clears the trace. Phases:
+ * 1 pass start, 2 partial sums done, 3 block reduction done, 4 message ready,
+ * 5 Fiat-Shamir hop done, 6 round bound. */
+int zs_ctx_trace(zs_ctx* ctx, int enable);
+int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
+                      uint64_t* clock_khz);
 
 /* ---- transcript (host) ------------------------------------------------
  * Replaces hash-frx's Sha256State stream under Sha256FieldTranscript

@@ -253,6 +253,7 @@ void zs_ctx_destroy(zs_ctx* ctx) {
     (void)hipEventDestroy(r.stop);
   }
   for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  if (ctx->trace) (void)hipFree(ctx->trace);
   delete ctx->host_pool;
   delete ctx->pool;
   (void)hipStreamDestroy(ctx->stream);
@@ -305,6 +306,45 @@ int zs_ctx_profile(zs_ctx* ctx, int enable) {
   }
   ctx->records.clear();
   ctx->profile = enable != 0;
+  return ZS_OK;
+}
+
+static const size_t kTraceWords = 4096;
+
+int zs_ctx_trace(zs_ctx* ctx, int enable) {
+  if (!ctx) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  if (enable && !ctx->trace) {
+    ZS_HIP(hipMalloc((void**)&ctx->trace, kTraceWords * sizeof(uint64_t)));
+  }
+  if (!enable && ctx->trace) {
+    ZS_HIP(hipFree(ctx->trace));
+    ctx->trace = nullptr;
+  }
+  if (ctx->trace) ZS_HIP(hipMemset(ctx->trace, 0, kTraceWords * sizeof(uint64_t)));
+  return ZS_OK;
+}
+
+int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
+                      uint64_t* clock_khz) {
+  if (!ctx || !count) return fail(ZS_EINVAL, "null argument");
+  *count = 0;
+  if (!ctx->trace) return ZS_OK;
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<uint64_t> buf(kTraceWords);
+  ZS_HIP(hipMemcpy(buf.data(), ctx->trace, kTraceWords * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  size_t n = buf[0] < (kTraceWords - 1) / 2 ? (size_t)buf[0] : (kTraceWords - 1) / 2;
+  if (n > cap) n = cap;
+  if (pairs) memcpy(pairs, buf.data() + 1, n * 2 * sizeof(uint64_t));
+  *count = n;
+  if (clock_khz) {
+    int rate = 0;
+    ZS_HIP(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, ctx->device));
+    *clock_khz = (uint64_t)rate;
+  }
+  ZS_HIP(hipMemset(ctx->trace, 0, kTraceWords * sizeof(uint64_t)));
   return ZS_OK;
 }
 

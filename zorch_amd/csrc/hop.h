@@ -36,6 +36,16 @@ struct WaveSha {
   uint64_t total;
 };
 
+// Keep a wave-uniform value in a vector register. Without this the compiler
+// runs uniform 32-bit arithmetic on the scalar unit, which lacks rotates and
+// 64-bit multiplies and shuttles every intermediate through readfirstlane — a
+// SHA-256 compression then takes ~3x longer than on the vector ALU.
+__device__ __forceinline__ uint32_t vreg(uint32_t x) {
+  uint32_t y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
 // Load the wave state from a Sha256State (global or LDS) into registers and
 // the pending bytes into the LDS block `pend` (64 B, 4-byte aligned).
 __device__ __forceinline__ void ws_load(WaveSha& s, const Sha256State& st, uint8_t* pend) {
@@ -142,10 +152,10 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
     uint32_t w[16], hh[8];
     const uint32_t* b32 = reinterpret_cast<const uint32_t*>(blk);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = __builtin_bswap32(b32[i]);
+    for (int i = 0; i < 16; ++i) w[i] = vreg(__builtin_bswap32(b32[i]));
     const bool from_dg = squeeze && dblk > 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) hh[i] = from_dg ? dg[i] : s.h[i];
+    for (int i = 0; i < 8; ++i) hh[i] = vreg(from_dg ? dg[i] : s.h[i]);
     sha256_compress_words(hh, w);
     wave_sync();
     if (!squeeze) {

@@ -87,6 +87,7 @@ class InstanceImpl final : public Instance {
     args_.eq_point = EQ ? (const TC*)(b + o_eqp) : nullptr;
     args_.eq_l_fam = EQ ? (const TC*)(b + o_eql) : nullptr;
     args_.eq_r_fam = EQ ? (const TC*)(b + o_eqr) : nullptr;
+    args_.trace = ctx->trace;
     hipStream_t s = ctx->stream;
     if (stage_in) {
       ZS_HIP(hipMemcpyAsync((void*)args_.in, a.tables, (size_t)K * N * sizeof(TT),

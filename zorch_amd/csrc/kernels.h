@@ -83,7 +83,21 @@ struct StageArgs {
   const typename FC::T* eq_point;  // n coordinates (eq mode)
   const typename FC::T* eq_l_fam;  // suffix family of eq_point[0 : n/2]
   const typename FC::T* eq_r_fam;  // suffix family of eq_point[n/2 : n]
+  uint64_t* trace;                 // phase timestamps (null: off)
 };
+
+// Phase trace: trace[0] counts events; event i is (round << 8 | slot,
+// wall-clock ticks) at trace[1 + 2i]. One lane records; off when null.
+__device__ __forceinline__ void trace_mark(uint64_t* trace, uint32_t round, uint32_t slot) {
+  if (trace) {
+    const uint64_t i = __hip_atomic_fetch_add(trace, (uint64_t)1, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if (i < 2047) {
+      trace[1 + 2 * i] = ((uint64_t)round << 8) | slot;
+      trace[2 + 2 * i] = wall_clock64();
+    }
+  }
+}
 
 // --------------------------------------------------- challenge multiplier
 // Multiplication by the round challenge r, with the X^4 = 11 wrap of r
@@ -326,6 +340,20 @@ __device__ __forceinline__ uint64_t el_from_bytes<GL>(const uint8_t* b) {
   return gl_canon(w);
 }
 
+// A field element in vector registers (see vreg in hop.h).
+template <class F>
+__device__ __forceinline__ typename F::T vreg_el(typename F::T v);
+template <>
+__device__ __forceinline__ uint32_t vreg_el<BB>(uint32_t v) { return vreg(v); }
+template <>
+__device__ __forceinline__ bb4 vreg_el<BB4>(bb4 v) {
+  return bb4_make(vreg(v.c[0]), vreg(v.c[1]), vreg(v.c[2]), vreg(v.c[3]));
+}
+template <>
+__device__ __forceinline__ uint64_t vreg_el<GL>(uint64_t v) {
+  return ((uint64_t)vreg((uint32_t)(v >> 32)) << 32) | vreg((uint32_t)v);
+}
+
 // LDS scratch of the finisher.
 template <class FC>
 struct FinishLds {
@@ -349,20 +377,21 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   constexpr int EB = sizeof(T);
   const uint32_t lane = threadIdx.x & 63;
   Ctl<FC>* ctl = a.ctl;
+  // Every value below is wave-uniform; vreg_el keeps the arithmetic on the
+  // vector ALU (64-bit multiplies and rotates exist only there).
   T tv[NPTS];
 #pragma unroll
-  for (int t = 0; t <= K; ++t) tv[t] = sums[t];
-  T w_i = FC::zero();
+  for (int t = 0; t <= K; ++t) tv[t] = vreg_el<FC>(sums[t]);
   T msg[NPTS];
   if constexpr (EQ) {
     // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i)
     // (coefficients precomputed in the control block).
     T ext = FC::zero();
 #pragma unroll
-    for (int i = 0; i <= K; ++i) ext = FC::add(ext, FC::mul(tv[i], ctl->ext_coef[i]));
+    for (int i = 0; i <= K; ++i) ext = FC::add(ext, FC::mul(tv[i], vreg_el<FC>(ctl->ext_coef[i])));
     tv[K + 1] = ext;
-    w_i = a.eq_point[round];
-    T eqp = ctl->eq_prev;
+    const T w_i = vreg_el<FC>(a.eq_point[round]);
+    const T eqp = vreg_el<FC>(ctl->eq_prev);
     T l1 = FC::mul(eqp, w_i);
     T l0 = FC::sub(eqp, l1);
     T ld = FC::sub(l1, l0);
@@ -385,34 +414,42 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   }
   wave_sync();
   WaveSha ws;
+  if (lane == 0) trace_mark(a.trace, round, 4);
   ws_load(ws, ctl->sha, sm.pend);
   ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
             ctl->item_bytes, ctl->limbs, sm.tail, sm.out);
   ws_store(ws, ctl->sha, sm.pend);
+  if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
-  const T r = el_from_bytes<FC>(sm.out);
-  // Lagrange basis at r over {0..NPTS-1}, one node at a time (few live values):
-  // L_t(r) = prod_{j != t} (r - j) / prod_{j != t} (t - j).
+  const T r = vreg_el<FC>(el_from_bytes<FC>(sm.out));
+  // Lagrange basis at r over {0..NPTS-1}: L_t(r) = prod_{j != t} (r - j) /
+  // prod_{j != t} (t - j), from prefix and suffix products of (r - j).
+  T diff[NPTS], pre[NPTS];
+#pragma unroll
+  for (int j = 0; j < NPTS; ++j) diff[j] = FC::sub(r, vreg_el<FC>(ctl->nodes[j]));
+  pre[0] = FC::one();
+#pragma unroll
+  for (int j = 1; j < NPTS; ++j) pre[j] = FC::mul(pre[j - 1], diff[j - 1]);
   T claim = FC::zero();
-#pragma unroll 1
-  for (int t = 0; t < NPTS; ++t) {
-    T num = ctl->lag_inv[t];
-#pragma unroll 1
-    for (int j = 0; j < NPTS; ++j)
-      if (j != t) num = FC::mul(num, FC::sub(r, ctl->nodes[j]));
-    claim = FC::add(claim, FC::mul(sm.msg[t], num));
+  T suf = FC::one();
+#pragma unroll
+  for (int t = NPTS - 1; t >= 0; --t) {
+    const T basis = FC::mul(FC::mul(pre[t], suf), vreg_el<FC>(ctl->lag_inv[t]));
+    claim = FC::add(claim, FC::mul(msg[t], basis));
+    suf = FC::mul(suf, diff[t]);
   }
   T eq_next = FC::zero();
   if constexpr (EQ) {
-    T one = FC::one();
-    const T wi = a.eq_point[round];
-    T ef = FC::add(FC::mul(r, wi), FC::mul(FC::sub(one, r), FC::sub(one, wi)));
-    eq_next = FC::mul(ctl->eq_prev, ef);
+    const T one = FC::one();
+    const T wi = vreg_el<FC>(a.eq_point[round]);
+    const T ef = FC::add(FC::mul(r, wi), FC::mul(FC::sub(one, r), FC::sub(one, wi)));
+    eq_next = FC::mul(vreg_el<FC>(ctl->eq_prev), ef);
   }
   if (lane == 0) {
     ctl->claim = claim;
     ctl->r = r;
     if constexpr (EQ) ctl->eq_prev = eq_next;
+    trace_mark(a.trace, round, 6);
   }
   if (round + 1 == a.n && a.final_out && lane < a.k) {
     const uint32_t q = lane;
@@ -452,6 +489,7 @@ __global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
   if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
   ChalMul<FC> cm;
   if constexpr (FOLD) cm.init(a.ctl->r);
+  if (blockIdx.x == 0 && threadIdx.x == 0) trace_mark(a.trace, round, 1);
   const size_t start = (size_t)blockIdx.x * PASS_THREADS + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * PASS_THREADS;
   pass_accumulate<FS, FC, K, EQ, FOLD>(src, src_stride, a.work, a.work_stride, L, cm, wl, wr,
@@ -469,6 +507,7 @@ __global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
   }
   __syncthreads();
   if (!is_last) return;
+  if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
   T red[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) red[t] = FC::zero();
@@ -510,6 +549,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
     if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
     ChalMul<FC> cm;
     if (round > 0) cm.init(round == first ? a.ctl->r : r_sh);
+    if (threadIdx.x == 0) trace_mark(a.trace, round, 1);
     if (round == 0) {
       pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
                                             wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
@@ -520,7 +560,9 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
       pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride, L, cm,
                                            wl, wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
     }
+    if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
     block_reduce<FC, NA, TAIL_THREADS>(acc, lds, out);
+    if (threadIdx.x == 0) trace_mark(a.trace, round, 3);
     if (threadIdx.x < 64) {
       finish_round<FS, FC, K, EQ>(a, round, out, a.work, a.in,
                                   round == 0 ? a.in_stride : a.work_stride, round == 0, sm);

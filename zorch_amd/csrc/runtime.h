@@ -83,6 +83,7 @@ struct zs_ctx {
   zs::HostPool* host_pool = nullptr;
   zs::TuneParams tune;
   bool profile = false;
+  uint64_t* trace = nullptr;  // device phase-trace buffer (null: off)
   std::vector<zs::KernelRecord> records;
   std::vector<hipEvent_t> spare_events;
   hipEvent_t event() {

@@ -52,6 +52,15 @@ ZS_HD uint32_t sha_rotr(uint32_t x, int n) {
 #endif
 }
 
+// Three-way XOR: one v_bitop3 (truth table 0x96) on the device.
+ZS_HD uint32_t sha_xor3(uint32_t a, uint32_t b, uint32_t c) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 // One compression of the 16 big-endian message words w (consumed).
 ZS_HD void sha256_compress_words(uint32_t h[8], uint32_t w[16]) {
   uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
@@ -62,15 +71,15 @@ ZS_HD void sha256_compress_words(uint32_t h[8], uint32_t w[16]) {
       wt = w[t];
     } else {
       uint32_t x = w[(t - 15) & 15], y = w[(t - 2) & 15];
-      uint32_t s0 = sha_rotr(x, 7) ^ sha_rotr(x, 18) ^ (x >> 3);
-      uint32_t s1 = sha_rotr(y, 17) ^ sha_rotr(y, 19) ^ (y >> 10);
+      uint32_t s0 = sha_xor3(sha_rotr(x, 7), sha_rotr(x, 18), x >> 3);
+      uint32_t s1 = sha_xor3(sha_rotr(y, 17), sha_rotr(y, 19), y >> 10);
       wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
       w[t & 15] = wt;
     }
-    uint32_t S1 = sha_rotr(e, 6) ^ sha_rotr(e, 11) ^ sha_rotr(e, 25);
+    uint32_t S1 = sha_xor3(sha_rotr(e, 6), sha_rotr(e, 11), sha_rotr(e, 25));
     uint32_t ch = (e & f) ^ (~e & g);
     uint32_t t1 = hh + S1 + ch + ZS_K256[t] + wt;
-    uint32_t S0 = sha_rotr(a, 2) ^ sha_rotr(a, 13) ^ sha_rotr(a, 22);
+    uint32_t S0 = sha_xor3(sha_rotr(a, 2), sha_rotr(a, 13), sha_rotr(a, 22));
     uint32_t mj = (a & b) ^ (c & (a ^ b));
     hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + S0 + mj;
   }

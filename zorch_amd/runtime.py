@@ -228,3 +228,20 @@ def profile_read(ctx: Context, cap: int = 1 << 16) -> list[dict]:
     kinds = {1: "pass", 2: "finish", 3: "tail"}
     return [{"round": r.round, "kind": kinds.get(r.kind, str(r.kind)), "grid": r.grid,
              "bytes": int(r.bytes), "ms": float(r.ms)} for r in recs[: n.value]]
+
+
+def trace(ctx: Context, enable: bool) -> None:
+    """Turn the in-kernel phase trace on or off (clears it)."""
+    _lib.check(_lib.lib().zs_ctx_trace(ctx.handle, 1 if enable else 0))
+
+
+def trace_read(ctx: Context, cap: int = 2047) -> list[tuple[int, int, float]]:
+    """Phase events since the last read as (round, phase, microseconds)."""
+    buf = np.zeros(2 * cap, dtype=np.uint64)
+    n = ctypes.c_size_t(0)
+    khz = ctypes.c_uint64(0)
+    _lib.check(_lib.lib().zs_ctx_trace_read(ctx.handle, buf.ctypes.data, cap, ctypes.byref(n),
+                                            ctypes.byref(khz)))
+    rate = max(int(khz.value), 1) / 1e3  # ticks per microsecond
+    ev = buf[: 2 * n.value].reshape(-1, 2)
+    return [(int(k) >> 8, int(k) & 0xFF, float(t) / rate) for k, t in ev]
