@@ -42,7 +42,7 @@ struct WaveSha {
 // SHA-256 compression then takes ~3x longer than on the vector ALU.
 __device__ __forceinline__ uint32_t vreg(uint32_t x) {
   uint32_t y;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
   return y;
 }
 

@@ -467,19 +467,23 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
 }
 
 // ------------------------------------------------------------- kernels
-// Multi-block pass of one round. Every block writes its partial node sums;
-// the last block to take a ticket reduces them and finishes the round.
+template <class FC, int NA>
+struct RoundLds {
+  typename FC::T lds[(PASS_THREADS / 64) * NA];
+  typename FC::T out[NA];
+  FinishLds<FC> sm;
+  uint32_t is_last;
+};
+
+// One block's share of a multi-block round: block `blk` of `nblk` accumulates
+// its pairs, writes its partial node sums, and the last block of the round to
+// take a ticket reduces all partials and finishes the round.
 template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
-__global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
-                                                            const typename FS::T* src,
-                                                            size_t src_stride, uint32_t L,
-                                                            uint32_t round) {
+__device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const typename FS::T* src,
+                                            size_t src_stride, uint32_t L, uint32_t round,
+                                            uint32_t blk, uint32_t nblk, RoundLds<FC, K + 1>& sh) {
   using T = typename FC::T;
   constexpr int NA = K + 1;
-  __shared__ T lds[(PASS_THREADS / 64) * NA];
-  __shared__ T out[NA];
-  __shared__ FinishLds<FC> sm;
-  __shared__ uint32_t is_last;
   T acc[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
@@ -489,41 +493,81 @@ __global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
   if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
   ChalMul<FC> cm;
   if constexpr (FOLD) cm.init(a.ctl->r);
-  if (blockIdx.x == 0 && threadIdx.x == 0) trace_mark(a.trace, round, 1);
-  const size_t start = (size_t)blockIdx.x * PASS_THREADS + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * PASS_THREADS;
+  if (blk == 0 && threadIdx.x == 0) trace_mark(a.trace, round, 1);
+  const size_t start = (size_t)blk * PASS_THREADS + threadIdx.x;
+  const size_t stride = (size_t)nblk * PASS_THREADS;
   pass_accumulate<FS, FC, K, EQ, FOLD>(src, src_stride, a.work, a.work_stride, L, cm, wl, wr,
                                        nxr_log, start, stride, acc);
-  block_reduce<FC, NA, PASS_THREADS>(acc, lds, out);
-  if (threadIdx.x < NA) store_coherent<FC>(a.partials + (size_t)blockIdx.x * NA + threadIdx.x,
-                                           out[threadIdx.x]);
-  // Partial stores complete before the ticket is taken.
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t prev =
-        __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
-  T red[NA];
+  block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
+  if (nblk > 1) {
+    if (threadIdx.x < NA) store_coherent<FC>(a.partials + (size_t)blk * NA + threadIdx.x,
+                                             sh.out[threadIdx.x]);
+    // Partial stores complete before the ticket is taken.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t prev =
+          __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sh.is_last = (prev == nblk - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!sh.is_last) return;
+    if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
+    T red[NA];
 #pragma unroll
-  for (int t = 0; t < NA; ++t) red[t] = FC::zero();
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += PASS_THREADS) {
+    for (int t = 0; t < NA; ++t) red[t] = FC::zero();
+    for (uint32_t b = threadIdx.x; b < nblk; b += PASS_THREADS) {
 #pragma unroll
-    for (int t = 0; t < NA; ++t)
-      red[t] = FC::add(red[t], load_coherent<FC>(a.partials + (size_t)b * NA + t));
+      for (int t = 0; t < NA; ++t)
+        red[t] = FC::add(red[t], load_coherent<FC>(a.partials + (size_t)b * NA + t));
+    }
+    block_reduce<FC, NA, PASS_THREADS>(red, sh.lds, sh.out);
   }
-  block_reduce<FC, NA, PASS_THREADS>(red, lds, out);
   if (threadIdx.x < 64) {
-    const T* last_work = a.work;
-    finish_round<FT, FC, K, EQ>(a, round, out, last_work, a.in,
-                                round == 0 ? a.in_stride : a.work_stride, round == 0, sm);
+    finish_round<FT, FC, K, EQ>(a, round, sh.out, a.work, a.in,
+                                round == 0 ? a.in_stride : a.work_stride, round == 0, sh.sm);
   }
-  if (threadIdx.x == 0)
+  if (nblk > 1 && threadIdx.x == 0)
     __hip_atomic_store(&a.ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Multi-block pass of one round of one proof.
+template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
+__global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
+                                                            const typename FS::T* src,
+                                                            size_t src_stride, uint32_t L,
+                                                            uint32_t round) {
+  __shared__ RoundLds<FC, K + 1> sh;
+  round_block<FT, FS, FC, K, EQ, FOLD>(a, src, src_stride, L, round, blockIdx.x, gridDim.x, sh);
+}
+
+// One round of many proofs in one launch: item i owns blocks
+// [block_off, block_off + blocks); `owner` maps each block to its item.
+template <class FT, class FC>
+struct BatchItem {
+  StageArgs<FT, FC> a;
+  uint32_t round;
+  uint32_t blocks;
+  uint32_t block_off;
+  uint32_t L;
+};
+
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) void batch_kernel(const BatchItem<FT, FC>* items,
+                                                             const uint32_t* owner) {
+  __shared__ RoundLds<FC, K + 1> sh;
+  const BatchItem<FT, FC>& it = items[owner[blockIdx.x]];
+  const uint32_t blk = blockIdx.x - it.block_off;
+  if (it.round == 0) {
+    round_block<FT, FT, FC, K, EQ, false>(it.a, it.a.in, it.a.in_stride, it.L, 0, blk, it.blocks,
+                                          sh);
+  } else if (it.round == 1) {
+    round_block<FT, FT, FC, K, EQ, true>(it.a, it.a.in, it.a.in_stride, it.L, 1, blk, it.blocks,
+                                         sh);
+  } else {
+    round_block<FT, FC, FC, K, EQ, true>(it.a, it.a.work, it.a.work_stride, it.L, it.round, blk,
+                                         it.blocks, sh);
+  }
 }
 
 // Single-block kernel for the small tail rounds [first, last): each round is
