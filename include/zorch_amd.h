@@ -135,6 +135,21 @@ int zs_ctx_trace(zs_ctx* ctx, int enable);
 int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
                       uint64_t* clock_khz);
 
+/* ---- launch shapes -------------------------------------------------------
+ * Tuning knobs with no counterpart in the reference (XLA picks its own
+ * launch shapes): rounds with at most tail_pairs pairs run in one single-block
+ * kernel; a multi-block pass uses a power-of-two grid of at most max_grid
+ * blocks (<= 4096) sized for pairs_per_thread pairs per thread. Results do not
+ * depend on them. Set applies to proofs started afterwards. */
+typedef struct zs_tune {
+  uint32_t tail_pairs;
+  uint32_t max_grid;
+  uint32_t pairs_per_thread;
+  uint32_t reserved;
+} zs_tune;
+int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out);
+int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* tune);
+
 /* ---- transcript (host) ------------------------------------------------
  * Replaces hash-frx's Sha256State stream under Sha256FieldTranscript
  * (zorch/sha256_field_transcript.py:45-50,100-114,178-181). Framing is

@@ -79,6 +79,15 @@ class KernelRecordC(ctypes.Structure):
     ]
 
 
+class TuneC(ctypes.Structure):
+    _fields_ = [
+        ("tail_pairs", ctypes.c_uint32),
+        ("max_grid", ctypes.c_uint32),
+        ("pairs_per_thread", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 assert ctypes.sizeof(Sha256StateC) == 112
 assert ctypes.sizeof(TranscriptC) == 128
 
@@ -102,6 +111,8 @@ SIGNATURES = {
     "zs_memcpy_dtoh": (_INT, [_P, _P, _P, _SZ]),
     "zs_ctx_trace": (_INT, [_P, _INT]),
     "zs_ctx_trace_read": (_INT, [_P, _P, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(_U64)]),
+    "zs_ctx_get_tune": (_INT, [_P, ctypes.POINTER(TuneC)]),
+    "zs_ctx_set_tune": (_INT, [_P, ctypes.POINTER(TuneC)]),
     "zs_ctx_profile": (_INT, [_P, _INT]),
     "zs_ctx_profile_read": (_INT, [_P, ctypes.POINTER(KernelRecordC), _U32,
                                    ctypes.POINTER(_U32)]),

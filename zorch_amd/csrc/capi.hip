@@ -296,6 +296,25 @@ int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return ZS_OK;
 }
 
+int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out) {
+  if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
+  *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread, 0};
+  return ZS_OK;
+}
+
+int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* t) {
+  if (!ctx || !t) return fail(ZS_EINVAL, "null argument");
+  const uint32_t g = t->max_grid;
+  if (g == 0 || g > zs::TUNE_MAX_GRID || (g & (g - 1)))
+    return fail(ZS_EINVAL, "max_grid must be a power of two in [1, 4096]");
+  if (t->pairs_per_thread == 0 || t->pairs_per_thread > (1u << 20))
+    return fail(ZS_EINVAL, "pairs_per_thread must be in [1, 2^20]");
+  ctx->tune.tail_pairs = t->tail_pairs;
+  ctx->tune.max_grid = g;
+  ctx->tune.pairs_per_thread = t->pairs_per_thread;
+  return ZS_OK;
+}
+
 int zs_ctx_profile(zs_ctx* ctx, int enable) {
   if (!ctx) return fail(ZS_EINVAL, "null argument");
   ZS_HIP(hipSetDevice(ctx->device));

@@ -46,29 +46,6 @@ __device__ __forceinline__ uint32_t vreg(uint32_t x) {
   return y;
 }
 
-// Load the wave state from a Sha256State (global or LDS) into registers and
-// the pending bytes into the LDS block `pend` (64 B, 4-byte aligned).
-__device__ __forceinline__ void ws_load(WaveSha& s, const Sha256State& st, uint8_t* pend) {
-  const uint32_t lane = threadIdx.x & 63;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) s.h[i] = st.h[i];
-  s.plen = (uint32_t)st.pending_len;
-  s.total = st.total_len;
-  pend[lane] = st.pending[lane];
-  wave_sync();
-}
-
-__device__ __forceinline__ void ws_store(const WaveSha& s, Sha256State& st, const uint8_t* pend) {
-  const uint32_t lane = threadIdx.x & 63;
-  st.pending[lane] = lane < s.plen ? pend[lane] : 0;
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) st.h[i] = s.h[i];
-    st.pending_len = s.plen;
-    st.total_len = s.total;
-  }
-}
-
 // Fill the padded tail of SHA256(stream || le64(ctr)) into `tail` (128 B of
 // LDS), one byte per lane; returns the number of tail blocks (1 or 2).
 __device__ __forceinline__ uint32_t ws_build_tail(const WaveSha& s, const uint8_t* pend,
@@ -93,10 +70,13 @@ __device__ __forceinline__ uint32_t ws_build_tail(const WaveSha& s, const uint8_
 
 // observe(msg) + sample(limbs): `msg` (LDS, msg_len bytes) is framed into
 // `buf` (LDS, >= msg_len + 20 bytes); the squeezed bytes land in `out` (LDS,
-// 32-byte granules), and `tail` (LDS, 128 B) holds squeeze tail blocks.
+// 32-byte granules), `tail` (LDS, 128 B) holds squeeze tail blocks and `dgw`
+// (LDS, 8 words) the chaining value between the two blocks of a squeeze
+// tail, so that only the stream state occupies registers across compressions.
 __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* buf,
                                           const uint8_t* msg, uint32_t msg_len, uint32_t item,
-                                          uint32_t limbs, uint8_t* tail, uint8_t* out) {
+                                          uint32_t limbs, uint8_t* tail, uint32_t* dgw,
+                                          uint8_t* out) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t count = msg_len / item;
   const uint32_t framed = msg_len + 20u;
@@ -118,9 +98,6 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
   const uint8_t* src = buf;
   uint32_t src_len = framed, src_pos = 0;
   uint32_t ctr = 0, dblk = 0, dnblk = 0, got = 0;
-  uint32_t dg[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) dg[i] = 0;
   while (phase != 3u) {
     const uint8_t* blk;
     bool squeeze;
@@ -155,7 +132,7 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
     for (int i = 0; i < 16; ++i) w[i] = vreg(__builtin_bswap32(b32[i]));
     const bool from_dg = squeeze && dblk > 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) hh[i] = vreg(from_dg ? dg[i] : s.h[i]);
+    for (int i = 0; i < 8; ++i) hh[i] = vreg(from_dg ? dgw[i] : s.h[i]);
     sha256_compress_words(hh, w);
     wave_sync();
     if (!squeeze) {
@@ -164,18 +141,16 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
       s.plen = 0;
       continue;
     }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dg[i] = hh[i];
-    if (++dblk < dnblk) continue;
-    // Digest complete: its first bytes are squeeze output.
-    const uint32_t take = (nbytes - got) < 32u ? (nbytes - got) : 32u;
-    uint32_t* tw = reinterpret_cast<uint32_t*>(tail);
     if (lane == 0) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) tw[i] = __builtin_bswap32(dg[i]);
+      for (int i = 0; i < 8; ++i) dgw[i] = hh[i];
     }
     wave_sync();
-    if (lane < take) out[got + lane] = tail[lane];
+    if (++dblk < dnblk) continue;
+    // Digest complete: its first bytes (big-endian words) are squeeze output.
+    const uint32_t take = (nbytes - got) < 32u ? (nbytes - got) : 32u;
+    if (lane < take)
+      out[got + lane] = (uint8_t)(dgw[lane >> 2] >> (24u - 8u * (lane & 3u)));
     wave_sync();
     got += take;
     ++ctr;

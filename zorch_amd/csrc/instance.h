@@ -3,8 +3,9 @@
 // inst_*.hip (one translation unit each, compiled in parallel).
 //
 // Device block layout (one pool allocation, 256-byte aligned regions):
-//   io region:  control block | messages | raw point | final values | eq point
-//   partials | eq_l family | eq_r family | folded tables | staged input tables
+//   io region:  control block | messages | raw point | final values | slot rows |
+//               eq point | eq line coefficients
+//   eq_l family | eq_r family | folded tables | staged input tables
 // The io region is mirrored by one pinned host block, so a proof costs one
 // host-to-device copy of the control block and eq point before its rounds and
 // one device-to-host copy of the outputs after them.
@@ -25,6 +26,8 @@ typename F::T load_host_el(const void* src, size_t i) {
   memcpy(&v, (const uint8_t*)src + i * sizeof(typename F::T), sizeof(v));
   return F::canon(v);
 }
+
+static_assert(TUNE_MAX_GRID == (uint32_t)MAXG, "grid limit");
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -53,10 +56,13 @@ class InstanceImpl final : public Instance {
     o_point_ = take((size_t)n_ * sizeof(TC));
     o_final_ = take((size_t)(K + 1) * sizeof(TC));
     out_bytes_ = off;
-    size_t o_eqp = 0, o_eql = 0, o_eqr = 0;
-    if (EQ) o_eqp = take((size_t)n_ * sizeof(TC));
+    const size_t o_slots = take((size_t)NSLOT * SLOTW * sizeof(uint64_t));
+    size_t o_eqp = 0, o_eqab = 0, o_eql = 0, o_eqr = 0;
+    if (EQ) {
+      o_eqp = take((size_t)n_ * sizeof(TC));
+      o_eqab = take((size_t)2 * n_ * sizeof(TC));
+    }
     io_bytes_ = off;
-    const size_t o_part = take((size_t)MAXG * NPTS * sizeof(TC));
     if (EQ) {
       o_eql = take(((size_t)2 << l_half) * sizeof(TC));
       o_eqr = take(((size_t)2 << (n_ - l_half)) * sizeof(TC));
@@ -80,11 +86,12 @@ class InstanceImpl final : public Instance {
     args_.n = n_;
     args_.k = K;
     args_.ctl = (Ctl<FC>*)(b + o_ctl_);
-    args_.partials = (TC*)(b + o_part);
+    args_.slots = (uint64_t*)(b + o_slots);
     args_.msgs = (TC*)(b + o_msgs_);
     args_.point = b + o_point_;
     args_.final_out = (TC*)(b + o_final_);
     args_.eq_point = EQ ? (const TC*)(b + o_eqp) : nullptr;
+    args_.eq_ab = EQ ? (const TC*)(b + o_eqab) : nullptr;
     args_.eq_l_fam = EQ ? (const TC*)(b + o_eql) : nullptr;
     args_.eq_r_fam = EQ ? (const TC*)(b + o_eqr) : nullptr;
     args_.trace = ctx->trace;
@@ -119,7 +126,12 @@ class InstanceImpl final : public Instance {
     }
     if (EQ) {
       TC* ep = (TC*)(h + o_eqp);
-      for (uint32_t i = 0; i < n_; ++i) ep[i] = load_host_el<FC>(a.eq_point, i);
+      TC* ab = (TC*)(h + o_eqab);
+      for (uint32_t i = 0; i < n_; ++i) {
+        ep[i] = load_host_el<FC>(a.eq_point, i);
+        ab[2 * i] = FC::sub(FC::add(ep[i], ep[i]), FC::one());  // 2 w_i - 1
+        ab[2 * i + 1] = FC::sub(FC::one(), ep[i]);              // 1 - w_i
+      }
     }
     ZS_HIP(hipMemcpyAsync(b, h, io_bytes_, hipMemcpyHostToDevice, s));
     if (EQ) {
@@ -177,16 +189,17 @@ class InstanceImpl final : public Instance {
       // A power-of-two grid keeps each thread's eq_r index fixed across its pairs.
       while (grid & (grid - 1)) grid &= grid - 1;
       const uint32_t L = 1u << (n_ - r);
+      const uint32_t need_claim = r + 1 == last ? 1u : 0u;  // the claim is read after `last`
       hipEvent_t ev = begin(s);
       if (r == 0) {
         pass_kernel<FT, FT, FC, K, EQ, false>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r);
+            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
       } else if (r == 1) {
         pass_kernel<FT, FT, FC, K, EQ, true>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r);
+            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
       } else {
         pass_kernel<FT, FC, FC, K, EQ, true>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.work, args_.work_stride, L, r);
+            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.work, args_.work_stride, L, r, need_claim);
       }
       ZS_HIP(hipGetLastError());
       if (ev) mark(s, r, KREC_PASS, (uint32_t)grid, pass_bytes(r), ev);

@@ -10,14 +10,17 @@
 //                                               (domain.py:131-137,158-205),
 //      with w the factored eq weight eq_l[j_hi] eq_r[j_lo] in eq mode
 //      (eq_poly.py:63-97,144-187), never materialised.
-// The last block to finish (a ticket counter) reduces the per-block partials,
-// extends t to node K+1 in eq mode (a degree-K polynomial is fixed by K+1
-// values, so t(K+1) = sum_i (-1)^(K-i) C(K+1, i) t(i) exactly), forms the
-// message s(t) = l(t) t(t) (eq_poly.py:100-111), runs the SHA-256 Fiat-Shamir
-// hop on one wavefront (hop.h), reduces the claim by Lagrange interpolation
-// (reduce.py:27-52, poly/univariate.py:36-65) and binds the challenge
-// (round.py:41-49). The challenge stays in device memory, so a whole stage is
-// a chain of launches on one stream with no host round trip.
+// Each block adds its node sums, word by word, into one of NSLOT 64-bit slot
+// rows with device-scope atomics (field words are < 2^32, so a slot never
+// overflows); the last block to finish (a ticket counter) sums the slot rows,
+// reduces the words mod p and finishes the round on one wavefront, one node
+// per lane: it extends t to node K+1 in eq mode (a degree-K polynomial is
+// fixed by K+1 values, so t(K+1) = sum_i (-1)^(K-i) C(K+1, i) t(i) exactly),
+// forms the message s(t) = l(t) t(t) (eq_poly.py:100-111), runs the SHA-256
+// Fiat-Shamir hop (hop.h), binds the challenge (round.py:41-49) and, where the
+// claim is read, reduces it by Lagrange interpolation (reduce.py:27-52,
+// poly/univariate.py:36-65). The challenge stays in device memory, so a whole
+// stage is a chain of launches on one stream with no host round trip.
 //
 // Layout: tables are table-major (table q at q*stride), element-contiguous;
 // BabyBear^4 elements are 16 B and move as one dwordx4. Pair j of a round with
@@ -48,7 +51,9 @@ constexpr int MAXPTS = 6;     // degree <= 4 product + eq factor
 constexpr int MAXK = 4;
 constexpr int PASS_THREADS = 256;
 constexpr int TAIL_THREADS = 1024;
-constexpr int MAXG = 4096;    // max blocks of one pass (partials capacity)
+constexpr int MAXG = 4096;    // max blocks of one pass
+constexpr int NSLOT = 32;     // slot rows of the cross-block node sums
+constexpr int SLOTW = 32;     // 64-bit words per slot row (>= (K+1) x words/element)
 
 // Device control block of one proof (lives in device memory).
 template <class FC>
@@ -76,11 +81,12 @@ struct StageArgs {
   uint32_t n;
   uint32_t k;
   Ctl<FC>* ctl;
-  typename FC::T* partials;     // MAXG x MAXPTS
+  uint64_t* slots;              // NSLOT x SLOTW, all zero between rounds
   typename FC::T* msgs;         // n x npts
   uint8_t* point;               // n x sizeof(FC::T) raw challenge bytes
   typename FC::T* final_out;    // k (+1) values, may be null
   const typename FC::T* eq_point;  // n coordinates (eq mode)
+  const typename FC::T* eq_ab;     // n x (2 w_i - 1, 1 - w_i): l_i(t) = eq_prev (B + t A)
   const typename FC::T* eq_l_fam;  // suffix family of eq_point[0 : n/2]
   const typename FC::T* eq_r_fam;  // suffix family of eq_point[n/2 : n]
   uint64_t* trace;                 // phase timestamps (null: off)
@@ -177,27 +183,47 @@ __device__ __forceinline__ void block_reduce(typename F::T (&acc)[N], typename F
   __syncthreads();
 }
 
-// Coherent (device-scope) word access for the cross-block partials.
+// Sum of the (lane & 7) groups of eight lanes: every lane of a group gets the
+// field sum of the group's values.
 template <class F>
-__device__ __forceinline__ void store_coherent(typename F::T* p, typename F::T v) {
-  constexpr int W = sizeof(typename F::T) / 4;
-  uint32_t w[W];
-  __builtin_memcpy(w, &v, sizeof(v));
-  uint32_t* d = reinterpret_cast<uint32_t*>(p);
+__device__ __forceinline__ typename F::T lanes8_sum(typename F::T v) {
 #pragma unroll
-  for (int i = 0; i < W; ++i) __hip_atomic_store(d + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class F>
-__device__ __forceinline__ typename F::T load_coherent(const typename F::T* p) {
-  constexpr int W = sizeof(typename F::T) / 4;
-  uint32_t w[W];
-  const uint32_t* s = reinterpret_cast<const uint32_t*>(p);
-#pragma unroll
-  for (int i = 0; i < W; ++i) w[i] = __hip_atomic_load(s + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  typename F::T v;
-  __builtin_memcpy(&v, w, sizeof(v));
+  for (int m = 1; m < 8; m <<= 1) v = F::add(v, shfl_xor_el<F>(v, m));
   return v;
 }
+
+// Cross-block node sums as 32-bit words added into 64-bit slots: an element
+// is W words, each below 2^32, and a slot total is reduced back to the field.
+// Sums of BabyBear Montgomery words are Montgomery words of the sum.
+__device__ __forceinline__ uint32_t bb_from_u64(uint64_t x) {  // x < p 2^32
+  return bb_mul(bb_redc(x), BB_R2);                             // (x / 2^32) 2^64 / 2^32
+}
+template <class F>
+struct SlotOps;
+template <>
+struct SlotOps<BB> {
+  static constexpr int W = 1;
+  static __device__ __forceinline__ uint32_t word(uint32_t v, int) { return v; }
+  static __device__ __forceinline__ uint32_t join(const uint64_t* s) { return bb_from_u64(s[0]); }
+};
+template <>
+struct SlotOps<BB4> {
+  static constexpr int W = 4;
+  static __device__ __forceinline__ uint32_t word(bb4 v, int c) { return v.c[c]; }
+  static __device__ __forceinline__ bb4 join(const uint64_t* s) {
+    return bb4_make(bb_from_u64(s[0]), bb_from_u64(s[1]), bb_from_u64(s[2]), bb_from_u64(s[3]));
+  }
+};
+template <>
+struct SlotOps<GL> {
+  static constexpr int W = 2;  // low and high halves
+  static __device__ __forceinline__ uint32_t word(uint64_t v, int c) {
+    return c ? (uint32_t)(v >> 32) : (uint32_t)v;
+  }
+  static __device__ __forceinline__ uint64_t join(const uint64_t* s) {  // s[i] < 2^44 < p
+    return gl_add(s[0], gl_mul(s[1], (uint64_t)1 << 32));
+  }
+};
 
 // ------------------------------------------------------------- eq weights
 // Round index i = round + 1 (reference numbering): both half tables while
@@ -227,6 +253,16 @@ __device__ inline void select_eq(uint32_t round, uint32_t n, const typename FC::
 // Accumulate this thread's pairs j = start, start+stride, ... < L/2 into
 // acc[t] = t(t), t = 0..K. FOLD: source tables have length 2L and are folded
 // at r into dst first; else the source tables are this round's tables.
+// PassUnroll pairs are loaded before any is folded or stored, so that a
+// thread keeps at least 64 bytes of loads in flight whatever the element size.
+// (Hoisting the loads is safe in place: the positions a thread stores, j and
+// j + L/2, are never positions it loads for another pair.)
+template <class FS, int K, bool FOLD>
+struct PassUnroll {
+  static constexpr int bytes = K * (FOLD ? 4 : 2) * (int)sizeof(typename FS::T);
+  static constexpr int value = bytes >= 64 ? 1 : bytes >= 32 ? 2 : 4;
+};
+
 template <class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void pass_accumulate(
     const typename FS::T* src, size_t src_stride, typename FC::T* dst, size_t dst_stride,
@@ -235,51 +271,86 @@ __device__ __forceinline__ void pass_accumulate(
   using FV = typename std::conditional<FOLD, FC, FS>::type;
   using V = typename FV::T;
   using T = typename FC::T;
+  using S = typename FS::T;
   constexpr bool WEIGHT_FIRST = EQ && std::is_same<FV, FC>::value;
+  constexpr int NS = FOLD ? 4 : 2;  // source streams per table
+  constexpr int U = PassUnroll<FS, K, FOLD>::value;
   const size_t half = L >> 1;
   const size_t nxr_mask = ((size_t)1 << nxr_log) - 1;
   const bool factored = EQ && wl != nullptr && (stride & nxr_mask) == 0;
-  for (size_t j = start; j < half; j += stride) {
-    V lo[K], df[K];
+  for (size_t j0 = start; j0 < half; j0 += (size_t)U * stride) {
+    S raw[U][K][NS];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-      const typename FS::T* s = src + (size_t)q * src_stride;
-      if constexpr (FOLD) {
-        typename FS::T s0 = s[j], s1 = s[j + L], s2 = s[j + half], s3 = s[j + half + L];
-        V a = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, FS::sub(s1, s0)));
-        V b = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, FS::sub(s3, s2)));
-        T* d = dst + (size_t)q * dst_stride;
-        d[j] = a;
-        d[j + half] = b;
-        lo[q] = a;
-        df[q] = FC::sub(b, a);
-      } else {
-        V a = s[j], b = s[j + half];
-        lo[q] = a;
-        df[q] = FV::sub(b, a);
+    for (int u = 0; u < U; ++u) {
+      const size_t j = j0 + (size_t)u * stride;
+      if (j < half) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const S* s = src + (size_t)q * src_stride;
+          raw[u][q][0] = s[j];
+          raw[u][q][1] = s[j + half];
+          if constexpr (FOLD) {
+            raw[u][q][2] = s[j + L];
+            raw[u][q][3] = s[j + half + L];
+          }
+        }
       }
     }
-    if constexpr (EQ) {
-      T w;
-      if (wl) {
-        w = wl[j >> nxr_log];
-        if (!factored) w = FC::mul(w, wr[j & nxr_mask]);
-      } else {
-        w = wr[j];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t j = j0 + (size_t)u * stride;
+      if (j >= half) break;
+      V lo[K], df[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        if constexpr (FOLD) {
+          const S s0 = raw[u][q][0], s2 = raw[u][q][1], s1 = raw[u][q][2], s3 = raw[u][q][3];
+          V a = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, FS::sub(s1, s0)));
+          V b = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, FS::sub(s3, s2)));
+          T* d = dst + (size_t)q * dst_stride;
+          d[j] = a;
+          d[j + half] = b;
+          lo[q] = a;
+          df[q] = FC::sub(b, a);
+        } else {
+          lo[q] = raw[u][q][0];
+          df[q] = FV::sub(raw[u][q][1], raw[u][q][0]);
+        }
       }
-      if constexpr (WEIGHT_FIRST) {
-        V hi0 = FV::add(lo[0], df[0]);
-        lo[0] = FC::mul(w, lo[0]);
-        df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
+      if constexpr (EQ) {
+        T w;
+        if (wl) {
+          w = wl[j >> nxr_log];
+          if (!factored) w = FC::mul(w, wr[j & nxr_mask]);
+        } else {
+          w = wr[j];
+        }
+        if constexpr (WEIGHT_FIRST) {
+          V hi0 = FV::add(lo[0], df[0]);
+          lo[0] = FC::mul(w, lo[0]);
+          df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
 #pragma unroll
-        for (int t = 0; t <= K; ++t) {
-          V prod = lo[0];
+          for (int t = 0; t <= K; ++t) {
+            V prod = lo[0];
 #pragma unroll
-          for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-          acc[t] = FC::add(acc[t], prod);
-          if (t < K) {
+            for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+            acc[t] = FC::add(acc[t], prod);
+            if (t < K) {
 #pragma unroll
-            for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+              for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t <= K; ++t) {
+            V prod = lo[0];
+#pragma unroll
+            for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+            acc[t] = FC::add(acc[t], MixMul<FC, FV>::go(w, prod));
+            if (t < K) {
+#pragma unroll
+              for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+            }
           }
         }
       } else {
@@ -288,25 +359,16 @@ __device__ __forceinline__ void pass_accumulate(
           V prod = lo[0];
 #pragma unroll
           for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-          acc[t] = FC::add(acc[t], MixMul<FC, FV>::go(w, prod));
+          acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
           if (t < K) {
 #pragma unroll
             for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
           }
         }
       }
-    } else {
-#pragma unroll
-      for (int t = 0; t <= K; ++t) {
-        V prod = lo[0];
-#pragma unroll
-        for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-        acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
-        if (t < K) {
-#pragma unroll
-          for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
-        }
-      }
+      // One pair's arithmetic at a time: interleaving the unrolled pairs
+      // would only raise register pressure (their loads are already issued).
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   if (factored && start < half) {
@@ -354,7 +416,9 @@ __device__ __forceinline__ uint64_t vreg_el<GL>(uint64_t v) {
   return ((uint64_t)vreg((uint32_t)(v >> 32)) << 32) | vreg((uint32_t)v);
 }
 
-// LDS scratch of the finisher.
+// LDS of the finisher: SHA-256 scratch, and the proof state between rounds
+// (stream midstate and counts, pending bytes, claim, challenge, eq mass), so
+// that only the hop's own working set occupies registers.
 template <class FC>
 struct FinishLds {
   alignas(16) uint8_t pend[64];
@@ -362,93 +426,129 @@ struct FinishLds {
   alignas(16) typename FC::T msg[MAXPTS];
   alignas(16) uint8_t buf[MAXPTS * 16 + 32];
   alignas(16) uint8_t out[64];
+  uint32_t dgw[8];
+  uint32_t h[8];
+  uint32_t plen;
+  uint64_t total;
+  typename FC::T claim, r, eq_prev;
 };
 
-// Wave 0 only (all 64 lanes, uniform values): message from the node sums,
-// Fiat-Shamir hop, claim reduction, bind, eq-mass update and (last round)
-// the final fold.
+// Control block -> finisher LDS (wave 0, all lanes).
+template <class FC>
+__device__ __forceinline__ void fin_load(FinishLds<FC>& sm, const Ctl<FC>* ctl) {
+  const uint32_t lane = threadIdx.x & 63;
+  sm.pend[lane] = ctl->sha.pending[lane];
+  if (lane < 8u) sm.h[lane] = ctl->sha.h[lane];
+  if (lane == 0) {
+    sm.plen = (uint32_t)ctl->sha.pending_len;
+    sm.total = ctl->sha.total_len;
+    sm.claim = ctl->claim;
+    sm.r = ctl->r;
+    sm.eq_prev = ctl->eq_prev;
+  }
+  wave_sync();
+}
+
+// Finisher LDS -> control block (wave 0, all lanes); pending bytes past the
+// pending length are stored as zero.
+template <class FC>
+__device__ __forceinline__ void fin_store(const FinishLds<FC>& sm, Ctl<FC>* ctl) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t plen = sm.plen;
+  ctl->sha.pending[lane] = lane < plen ? sm.pend[lane] : 0;
+  if (lane < 8u) ctl->sha.h[lane] = sm.h[lane];
+  if (lane == 0) {
+    ctl->sha.pending_len = plen;
+    ctl->sha.total_len = sm.total;
+    ctl->claim = sm.claim;
+    ctl->r = sm.r;
+    ctl->eq_prev = sm.eq_prev;
+  }
+}
+
+// Wave 0 only. Lane t = lane & 7 works on node t (the eight-lane groups of
+// the wave agree): message from the node sums, Fiat-Shamir hop, bind, eq-mass
+// update, the claim reduction when `need_claim`, and (last round) the final
+// fold. The proof state is read from and written back to `sm`.
 template <class FS, class FC, int K, bool EQ>
 __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
-                             const typename FC::T* sums, const typename FC::T* last_lo_hi_src,
-                             const typename FS::T* last_in, size_t last_stride,
-                             bool last_src_is_input, FinishLds<FC>& sm) {
+                                             bool need_claim, const typename FC::T* sums,
+                                             FinishLds<FC>& sm,
+                                             const typename FC::T* last_lo_hi_src,
+                                             const typename FS::T* last_in, size_t last_stride,
+                                             bool last_src_is_input) {
   using T = typename FC::T;
   constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
   constexpr int EB = sizeof(T);
+  static_assert(NPTS <= 8, "one node per lane of an eight-lane group");
   const uint32_t lane = threadIdx.x & 63;
-  Ctl<FC>* ctl = a.ctl;
-  // Every value below is wave-uniform; vreg_el keeps the arithmetic on the
-  // vector ALU (64-bit multiplies and rotates exist only there).
-  T tv[NPTS];
-#pragma unroll
-  for (int t = 0; t <= K; ++t) tv[t] = vreg_el<FC>(sums[t]);
-  T msg[NPTS];
-  if constexpr (EQ) {
-    // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i)
-    // (coefficients precomputed in the control block).
-    T ext = FC::zero();
-#pragma unroll
-    for (int i = 0; i <= K; ++i) ext = FC::add(ext, FC::mul(tv[i], vreg_el<FC>(ctl->ext_coef[i])));
-    tv[K + 1] = ext;
-    const T w_i = vreg_el<FC>(a.eq_point[round]);
-    const T eqp = vreg_el<FC>(ctl->eq_prev);
-    T l1 = FC::mul(eqp, w_i);
-    T l0 = FC::sub(eqp, l1);
-    T ld = FC::sub(l1, l0);
-    T lt = l0;
-#pragma unroll
-    for (int t = 0; t < NPTS; ++t) {
-      msg[t] = FC::mul(lt, tv[t]);
-      lt = FC::add(lt, ld);
+  const uint32_t t = lane & 7;
+  const uint32_t tn = t < (uint32_t)NPTS ? t : (uint32_t)(NPTS - 1);  // in-bounds node index
+  const Ctl<FC>* ctl = a.ctl;
+  {
+    T tv = sums[t <= (uint32_t)K ? t : (uint32_t)K];
+    if (t > (uint32_t)K) tv = FC::zero();
+    T msg;
+    if constexpr (EQ) {
+      const T A = a.eq_ab[2 * round], B = a.eq_ab[2 * round + 1];
+      const T node = ctl->nodes[tn];
+      const T coef = ctl->ext_coef[t <= (uint32_t)K ? t : 0u];
+      // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i).
+      const T ext = lanes8_sum<FC>(t <= (uint32_t)K ? FC::mul(tv, coef) : FC::zero());
+      if (t == (uint32_t)(K + 1)) tv = ext;
+      // l(t) = eq_prev ((1 - w) + t (2 w - 1)); lanes past the nodes carry zero.
+      msg = FC::mul(FC::mul(sm.eq_prev, FC::add(B, FC::mul(node, A))), tv);
+    } else {
+      msg = tv;
     }
-  } else {
-#pragma unroll
-    for (int t = 0; t < NPTS; ++t) msg[t] = tv[t];
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int t = 0; t < NPTS; ++t) {
-      a.msgs[(size_t)round * NPTS + t] = msg[t];
-      sm.msg[t] = msg[t];
+    if (lane < (uint32_t)NPTS) {
+      a.msgs[(size_t)round * NPTS + lane] = msg;
+      sm.msg[lane] = msg;
     }
   }
   wave_sync();
-  WaveSha ws;
   if (lane == 0) trace_mark(a.trace, round, 4);
-  ws_load(ws, ctl->sha, sm.pend);
-  ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
-            ctl->item_bytes, ctl->limbs, sm.tail, sm.out);
-  ws_store(ws, ctl->sha, sm.pend);
+  {
+    WaveSha ws;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ws.h[i] = sm.h[i];
+    ws.plen = sm.plen;
+    ws.total = sm.total;
+    ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg),
+              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, sm.tail, sm.dgw, sm.out);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) sm.h[i] = ws.h[i];
+      sm.plen = ws.plen;
+      sm.total = ws.total;
+    }
+  }
   if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
-  const T r = vreg_el<FC>(el_from_bytes<FC>(sm.out));
-  // Lagrange basis at r over {0..NPTS-1}: L_t(r) = prod_{j != t} (r - j) /
-  // prod_{j != t} (t - j), from prefix and suffix products of (r - j).
-  T diff[NPTS], pre[NPTS];
-#pragma unroll
-  for (int j = 0; j < NPTS; ++j) diff[j] = FC::sub(r, vreg_el<FC>(ctl->nodes[j]));
-  pre[0] = FC::one();
-#pragma unroll
-  for (int j = 1; j < NPTS; ++j) pre[j] = FC::mul(pre[j - 1], diff[j - 1]);
-  T claim = FC::zero();
-  T suf = FC::one();
-#pragma unroll
-  for (int t = NPTS - 1; t >= 0; --t) {
-    const T basis = FC::mul(FC::mul(pre[t], suf), vreg_el<FC>(ctl->lag_inv[t]));
-    claim = FC::add(claim, FC::mul(msg[t], basis));
-    suf = FC::mul(suf, diff[t]);
-  }
+  const T r = el_from_bytes<FC>(sm.out);
   T eq_next = FC::zero();
   if constexpr (EQ) {
-    const T one = FC::one();
-    const T wi = vreg_el<FC>(a.eq_point[round]);
-    const T ef = FC::add(FC::mul(r, wi), FC::mul(FC::sub(one, r), FC::sub(one, wi)));
-    eq_next = FC::mul(vreg_el<FC>(ctl->eq_prev), ef);
+    const T A = a.eq_ab[2 * round], B = a.eq_ab[2 * round + 1];
+    eq_next = FC::mul(sm.eq_prev, FC::add(B, FC::mul(r, A)));
   }
+  T claim = FC::zero();
+  if (need_claim) {
+    // s(r) = sum_t s(t) L_t(r) with L_t(r) = lag_inv[t] prod_{j != t} (r - j):
+    // lane t forms term t; lanes past the nodes add zero.
+    T basis = ctl->lag_inv[tn];
+#pragma unroll
+    for (int j = 0; j < NPTS; ++j) {
+      const T d = FC::sub(r, ctl->nodes[j]);
+      basis = FC::mul(basis, t == (uint32_t)j ? FC::one() : d);
+    }
+    const T msg = t < (uint32_t)NPTS ? sm.msg[tn] : FC::zero();
+    claim = lanes8_sum<FC>(FC::mul(msg, basis));
+  }
+  wave_sync();
   if (lane == 0) {
-    ctl->claim = claim;
-    ctl->r = r;
-    if constexpr (EQ) ctl->eq_prev = eq_next;
+    sm.r = r;
+    if (need_claim) sm.claim = claim;
+    if constexpr (EQ) sm.eq_prev = eq_next;
     trace_mark(a.trace, round, 6);
   }
   if (round + 1 == a.n && a.final_out && lane < a.k) {
@@ -464,6 +564,7 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
     a.final_out[q] = FC::add(lo, FC::mul(r, FC::sub(hi, lo)));
     if (EQ && q == 0) a.final_out[a.k] = eq_next;
   }
+  wave_sync();
 }
 
 // ------------------------------------------------------------- kernels
@@ -471,17 +572,20 @@ template <class FC, int NA>
 struct RoundLds {
   typename FC::T lds[(PASS_THREADS / 64) * NA];
   typename FC::T out[NA];
+  uint64_t sum64[PASS_THREADS / SLOTW][SLOTW];
   FinishLds<FC> sm;
   uint32_t is_last;
 };
 
 // One block's share of a multi-block round: block `blk` of `nblk` accumulates
-// its pairs, writes its partial node sums, and the last block of the round to
-// take a ticket reduces all partials and finishes the round.
+// its pairs and adds its node sums into slot row blk % NSLOT; the last block
+// of the round to take a ticket sums the rows, clears them and finishes the
+// round.
 template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const typename FS::T* src,
                                             size_t src_stride, uint32_t L, uint32_t round,
-                                            uint32_t blk, uint32_t nblk, RoundLds<FC, K + 1>& sh) {
+                                            bool need_claim, uint32_t blk, uint32_t nblk,
+                                            RoundLds<FC, K + 1>& sh) {
   using T = typename FC::T;
   constexpr int NA = K + 1;
   T acc[NA];
@@ -500,9 +604,16 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
                                        nxr_log, start, stride, acc);
   block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
   if (nblk > 1) {
-    if (threadIdx.x < NA) store_coherent<FC>(a.partials + (size_t)blk * NA + threadIdx.x,
-                                             sh.out[threadIdx.x]);
-    // Partial stores complete before the ticket is taken.
+    constexpr int W = SlotOps<FC>::W;
+    constexpr int NW = NA * W;
+    constexpr int G = PASS_THREADS / SLOTW;
+    static_assert(NW <= SLOTW && NSLOT % G == 0, "slot row layout");
+    if (threadIdx.x < NW) {
+      uint64_t* slot = a.slots + (size_t)(blk & (NSLOT - 1)) * SLOTW + threadIdx.x;
+      const uint32_t word = SlotOps<FC>::word(sh.out[threadIdx.x / W], threadIdx.x % W);
+      __hip_atomic_fetch_add(slot, (uint64_t)word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // The slot additions complete before the ticket is taken.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -513,32 +624,61 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
     __syncthreads();
     if (!sh.is_last) return;
     if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
-    T red[NA];
+    if (threadIdx.x < 64) fin_load(sh.sm, a.ctl);
+    // Thread (g, w) sums word w of rows g, g + G, ...
+    const uint32_t w = threadIdx.x % SLOTW, g = threadIdx.x / SLOTW;
+    uint64_t s = 0;
+    if (w < (uint32_t)NW) {
 #pragma unroll
-    for (int t = 0; t < NA; ++t) red[t] = FC::zero();
-    for (uint32_t b = threadIdx.x; b < nblk; b += PASS_THREADS) {
-#pragma unroll
-      for (int t = 0; t < NA; ++t)
-        red[t] = FC::add(red[t], load_coherent<FC>(a.partials + (size_t)b * NA + t));
+      for (int i = 0; i < NSLOT / G; ++i)
+        s += __hip_atomic_load(a.slots + (size_t)(g + i * G) * SLOTW + w, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
-    block_reduce<FC, NA, PASS_THREADS>(red, sh.lds, sh.out);
+    sh.sum64[g][w] = s;
+    __syncthreads();
+    // Every row has been read: clear them for the next round.
+    if (w < (uint32_t)NW) {
+#pragma unroll
+      for (int i = 0; i < NSLOT / G; ++i)
+        __hip_atomic_store(a.slots + (size_t)(g + i * G) * SLOTW + w, (uint64_t)0,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < NA) {
+      uint64_t tot[W];
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int q = 0; q < G; ++q) v += sh.sum64[q][threadIdx.x * W + c];
+        tot[c] = v;
+      }
+      sh.out[threadIdx.x] = SlotOps<FC>::join(tot);
+    }
+    __syncthreads();
+  } else if (threadIdx.x < 64) {
+    fin_load(sh.sm, a.ctl);
   }
   if (threadIdx.x < 64) {
-    finish_round<FT, FC, K, EQ>(a, round, sh.out, a.work, a.in,
-                                round == 0 ? a.in_stride : a.work_stride, round == 0, sh.sm);
+    finish_round<FT, FC, K, EQ>(a, round, need_claim, sh.out, sh.sm, a.work, a.in,
+                                round == 0 ? a.in_stride : a.work_stride, round == 0);
+    fin_store(sh.sm, a.ctl);
   }
   if (nblk > 1 && threadIdx.x == 0)
     __hip_atomic_store(&a.ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Multi-block pass of one round of one proof.
+// Multi-block pass of one round of one proof. The streaming loop needs fewer
+// registers than the finisher that one block runs at the end; PASS_WAVES_PER_EU
+// keeps the occupancy of the loop (the finisher may spill a few registers).
+#define PASS_WAVES_PER_EU __attribute__((amdgpu_waves_per_eu(5, 8)))
 template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
-__global__ __launch_bounds__(PASS_THREADS) void pass_kernel(StageArgs<FT, FC> a,
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(StageArgs<FT, FC> a,
                                                             const typename FS::T* src,
                                                             size_t src_stride, uint32_t L,
-                                                            uint32_t round) {
+                                                            uint32_t round, uint32_t need_claim) {
   __shared__ RoundLds<FC, K + 1> sh;
-  round_block<FT, FS, FC, K, EQ, FOLD>(a, src, src_stride, L, round, blockIdx.x, gridDim.x, sh);
+  round_block<FT, FS, FC, K, EQ, FOLD>(a, src, src_stride, L, round, need_claim != 0, blockIdx.x,
+                                       gridDim.x, sh);
 }
 
 // One round of many proofs in one launch: item i owns blocks
@@ -553,26 +693,28 @@ struct BatchItem {
 };
 
 template <class FT, class FC, int K, bool EQ>
-__global__ __launch_bounds__(PASS_THREADS) void batch_kernel(const BatchItem<FT, FC>* items,
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(const BatchItem<FT, FC>* items,
                                                              const uint32_t* owner) {
   __shared__ RoundLds<FC, K + 1> sh;
   const BatchItem<FT, FC>& it = items[owner[blockIdx.x]];
   const uint32_t blk = blockIdx.x - it.block_off;
+  const bool need_claim = it.round + 1 == it.a.n;
   if (it.round == 0) {
-    round_block<FT, FT, FC, K, EQ, false>(it.a, it.a.in, it.a.in_stride, it.L, 0, blk, it.blocks,
-                                          sh);
+    round_block<FT, FT, FC, K, EQ, false>(it.a, it.a.in, it.a.in_stride, it.L, 0, need_claim, blk,
+                                          it.blocks, sh);
   } else if (it.round == 1) {
-    round_block<FT, FT, FC, K, EQ, true>(it.a, it.a.in, it.a.in_stride, it.L, 1, blk, it.blocks,
-                                         sh);
-  } else {
-    round_block<FT, FC, FC, K, EQ, true>(it.a, it.a.work, it.a.work_stride, it.L, it.round, blk,
+    round_block<FT, FT, FC, K, EQ, true>(it.a, it.a.in, it.a.in_stride, it.L, 1, need_claim, blk,
                                          it.blocks, sh);
+  } else {
+    round_block<FT, FC, FC, K, EQ, true>(it.a, it.a.work, it.a.work_stride, it.L, it.round,
+                                         need_claim, blk, it.blocks, sh);
   }
 }
 
 // Single-block kernel for the small tail rounds [first, last): each round is
 // a block-stride pass, a block reduction and the finisher, separated by
-// barriers. Global writes of one round are visible to the block in the next.
+// barriers. Global writes of one round are visible to the block in the next;
+// the transcript, challenge and eq mass stay with wave 0 until the end.
 template <class FS, class FC, int K, bool EQ>
 __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a, uint32_t first,
                                                             uint32_t last) {
@@ -581,7 +723,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
   __shared__ T lds[(TAIL_THREADS / 64) * NA];
   __shared__ T out[NA];
   __shared__ FinishLds<FC> sm;
-  __shared__ T r_sh;
+  if (threadIdx.x < 64) fin_load(sm, a.ctl);
+  __syncthreads();
   for (uint32_t round = first; round < last; ++round) {
     const uint32_t L = 1u << (a.n - round);
     T acc[NA];
@@ -592,7 +735,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
     uint32_t nxr_log = 0;
     if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
     ChalMul<FC> cm;
-    if (round > 0) cm.init(round == first ? a.ctl->r : r_sh);
+    if (round > 0) cm.init(sm.r);
     if (threadIdx.x == 0) trace_mark(a.trace, round, 1);
     if (round == 0) {
       pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
@@ -608,12 +751,12 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
     block_reduce<FC, NA, TAIL_THREADS>(acc, lds, out);
     if (threadIdx.x == 0) trace_mark(a.trace, round, 3);
     if (threadIdx.x < 64) {
-      finish_round<FS, FC, K, EQ>(a, round, out, a.work, a.in,
-                                  round == 0 ? a.in_stride : a.work_stride, round == 0, sm);
-      if (threadIdx.x == 0) r_sh = a.ctl->r;
+      finish_round<FS, FC, K, EQ>(a, round, round + 1 == last, out, sm, a.work, a.in,
+                                  round == 0 ? a.in_stride : a.work_stride, round == 0);
     }
     __syncthreads();
   }
+  if (threadIdx.x < 64) fin_store(sm, a.ctl);
 }
 
 // Suffix eq families of the two halves of the eq point (compute_eq_evaluations,

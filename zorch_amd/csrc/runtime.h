@@ -60,6 +60,8 @@ class HostPool {
   std::map<void*, size_t> sizes_;
 };
 
+constexpr uint32_t TUNE_MAX_GRID = 4096;  // equals the kernels' MAXG
+
 struct TuneParams {
   uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
   uint32_t max_grid = 2048;     // blocks of a multi-block pass (<= MAXG)

@@ -39,6 +39,23 @@ class Context:
         _lib.check(_lib.lib().zs_ctx_reserved_bytes(self.handle, ctypes.byref(out)))
         return int(out.value)
 
+    def get_tune(self) -> dict:
+        """Launch shapes of later proofs: tail_pairs, max_grid, pairs_per_thread."""
+        t = _lib.TuneC()
+        _lib.check(_lib.lib().zs_ctx_get_tune(self.handle, ctypes.byref(t)))
+        return {"tail_pairs": t.tail_pairs, "max_grid": t.max_grid,
+                "pairs_per_thread": t.pairs_per_thread}
+
+    def set_tune(self, **kw) -> None:
+        """Change launch shapes (results never depend on them)."""
+        cur = self.get_tune()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning keys {sorted(unknown)}")
+        cur.update(kw)
+        t = _lib.TuneC(cur["tail_pairs"], cur["max_grid"], cur["pairs_per_thread"], 0)
+        _lib.check(_lib.lib().zs_ctx_set_tune(self.handle, ctypes.byref(t)))
+
     def close(self) -> None:
         if self._h is not None:
             _lib.lib().zs_ctx_destroy(self._h)
