@@ -108,6 +108,9 @@ int zs_device_alloc(zs_ctx* ctx, size_t bytes, void** dptr);
 int zs_device_free(zs_ctx* ctx, void* dptr);
 int zs_memcpy_htod(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
 int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy on the context's stream (synchronous); also the plain
+ * copy of known size that HBM byte counters are calibrated against. */
+int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- kernel timing -------------------------------------------------------
  * Profiling aid with no counterpart in the reference (whose only tracing is

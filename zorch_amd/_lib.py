@@ -109,6 +109,7 @@ SIGNATURES = {
     "zs_device_free": (_INT, [_P, _P]),
     "zs_memcpy_htod": (_INT, [_P, _P, _P, _SZ]),
     "zs_memcpy_dtoh": (_INT, [_P, _P, _P, _SZ]),
+    "zs_memcpy_dtod": (_INT, [_P, _P, _P, _SZ]),
     "zs_ctx_trace": (_INT, [_P, _INT]),
     "zs_ctx_trace_read": (_INT, [_P, _P, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(_U64)]),
     "zs_ctx_get_tune": (_INT, [_P, ctypes.POINTER(TuneC)]),

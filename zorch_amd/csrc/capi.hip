@@ -296,6 +296,14 @@ int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return ZS_OK;
 }
 
+int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(ZS_EINVAL, "null argument");
+  ZS_HIP(hipSetDevice(ctx->device));
+  ZS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  return ZS_OK;
+}
+
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out) {
   if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
   *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread, 0};
