@@ -105,6 +105,94 @@ def copy_calibration(nbytes: int, reps: int) -> None:
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def _verify_proof(inst, msgs, t0) -> bool:
+    from zorch_amd.challenge import ChallengePolicy
+    from zorch_amd.fields import FieldArray
+    from zorch_amd.sumcheck.eq.stage import EqPolyVerifier, EqSumClaim
+    from zorch_amd.sumcheck.prover import ProductSummand
+    from zorch_amd.sumcheck.stage import SumClaim, SumcheckVerifier
+    from zorch_amd.sumcheck.verifier import SumcheckRound
+    proof = FieldArray(msgs, inst.chal)
+    pol = ChallengePolicy(inst.chal)
+    if inst.mode == "zerocheck":
+        return EqPolyVerifier(ProductSummand(inst.degree), challenges=pol).verify(
+            EqSumClaim(inst.z, inst.claim, inst.n), proof, t0).ok
+    return SumcheckVerifier(SumcheckRound(inst.degree, pol)).verify(
+        SumClaim(inst.claim, inst.n), proof, t0).ok
+
+
+def bench_batch(args) -> dict:
+    """512 independent instances (sizes 2^12..2^22, product or zerocheck):
+    one zs_prove_batch call against 512 sequential zs_prove calls."""
+    from zorch_amd import _lib
+    from zorch_amd import workloads as W
+    from zorch_amd.runtime import DeviceTables, default_context
+    from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+    from zorch_amd.sumcheck._device import make_job
+
+    spec = W.batch_spec(count=args.batch_count)
+    insts = [W.batch_instance(spec, i) for i in range(len(spec.sizes))]
+    ctx = default_context()
+    lib = _lib.lib()
+    dev = [DeviceTables.from_host(i.tables, ctx) for i in insts]
+    modes = [_lib.ZS_MODE_EQ if i.mode == "zerocheck" else _lib.ZS_MODE_PRODUCT for i in insts]
+    jobs = [make_job(d, i.claim, i.chal, m, i.degree, i.z) for d, i, m in zip(dev, insts, modes)]
+    t0s = [Sha256FieldTranscript.new(i.label, i.transcript_field) for i in insts]
+    n = len(jobs)
+    args_arr = (_lib.ProveArgsC * n)(*[j.args for j in jobs])
+    proofs_arr = (_lib.ProofC * n)(*[j.proof for j in jobs])
+
+    def batched():
+        sts = (_lib.TranscriptC * n)(*[t.state.copy() for t in t0s])
+        _lib.check(lib.zs_prove_batch(ctx.handle, n, args_arr, sts, proofs_arr))
+
+    def plain():
+        for j, t in zip(jobs, t0s):
+            st = t.state.copy()
+            _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(j.args), ctypes.byref(st),
+                                    ctypes.byref(j.proof)))
+
+    plain()
+    plain_msgs = [j.msgs.copy() for j in jobs]
+    t = time.perf_counter()
+    reps_plain = 2
+    for _ in range(reps_plain):
+        plain()
+    plain_s = (time.perf_counter() - t) / reps_plain
+    for _ in range(max(args.warmup, 1)):
+        batched()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    batch_s = (time.perf_counter() - t) / args.steps
+    identical = all((j.msgs == m).all() for j, m in zip(jobs, plain_msgs))
+    verified = all(_verify_proof(i, j.msgs.copy(), t0) for i, j, t0 in zip(insts, jobs, t0s))
+    elems = sum(i.table_elems for i in insts)
+    alg = sum(W.algorithmic_bytes(i.tables.shape[0], i.n, i.tables.field.itemsize,
+                                  i.chal.itemsize) for i in insts)
+    for d in dev:
+        d.free()
+    return {
+        "metric": "batched sumcheck table-elems/s (512 instances, n ~ U{12..22})",
+        "value": elems / batch_s, "unit": "table-elems/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": batch_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 (BabyBear Montgomery / BabyBear^4)",
+        "data": "synthetic (seeded uniform field elements)",
+        "config": {"workload": f"batch_{n}", "instances": n, "sizes": "U{12..22} seed 4",
+                   "kinds": {"product": spec.kinds.count("product"),
+                             "zerocheck": spec.kinds.count("zerocheck")},
+                   "table_field": "babybear_mont", "challenge_field": "babybearx4_mont",
+                   "parallelism": "single GPU"},
+        "plain_ms": plain_s * 1e3, "plain_value": elems / plain_s,
+        "speedup_vs_plain": plain_s / batch_s,
+        "proofs_per_s": n / batch_s, "plain_proofs_per_s": n / plain_s,
+        "hbm_gbs": alg / batch_s / 1e9, "algorithmic_bytes": alg,
+        "identical_to_plain": bool(identical), "verified": bool(verified),
+        "tune": ctx.get_tune(),
+    }
+
+
 def cpu_baseline(inst, threads: int) -> dict:
     """The C oracle (the reference algorithm restated) on the host cores."""
     from tests.oracle_glue import run_c
@@ -130,6 +218,7 @@ def main() -> int:
     ap.add_argument("--workload", default="zerocheck_bb4_n24")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -139,6 +228,9 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, max(args.steps, 1))
+        return 0
+    if args.workload == "batch_512":
+        print(json.dumps(bench_batch(args)), flush=True)
         return 0
 
     from zorch_amd import _lib

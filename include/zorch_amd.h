@@ -175,8 +175,13 @@ int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]);
  * the Fiat-Shamir hop on the device; `transcript` is read and advanced. */
 int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
              zs_proof* out);
-/* Many independent instances (uneven sizes) in one submission; instance i
- * uses args[i], transcripts[i], outs[i]. */
+/* Many independent instances (uneven sizes, mixed kinds) in one submission;
+ * instance i uses args[i], transcripts[i], outs[i], with the same results as
+ * zs_prove on each. Instances of one kind (field pair, factor count, mode)
+ * advance together: step s is one launch that runs round s of every instance
+ * still running, so the per-round latency is paid once per step for the whole
+ * batch. One upload of all control blocks, one download of all outputs.
+ * The reference proves instances one at a time (no batched entry point). */
 int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
                    zs_transcript* transcripts, zs_proof* outs);
 

@@ -2,8 +2,11 @@
 // proving, batched proving and the host verifier step.
 #include <string.h>
 
+#include <algorithm>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/zorch_amd.h"
@@ -139,6 +142,14 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
   }
   *npts = a.degree + 1 + (a.mode == ZS_MODE_EQ ? 1 : 0);
   return ZS_OK;
+}
+
+BatchGroup* make_batch_group(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_batch_group_bb(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_batch_group_bb_bb4(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_batch_group_bb4(k, npts, eq);
+  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_batch_group_gl(k, npts, eq);
+  return nullptr;
 }
 
 Instance* make_instance(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
@@ -454,22 +465,35 @@ int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
   if (!ctx || (count && (!args || !transcripts || !outs))) return fail(ZS_EINVAL, "null argument");
   int rc;
   if ((rc = set_device(ctx))) return rc;
-  std::vector<std::unique_ptr<Instance>> insts(count);
+  // One group per kind (field pair, factor count, mode); every step launches
+  // the round of each group's members that are still running.
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, bool>, std::unique_ptr<BatchGroup>>
+      groups;
   for (size_t i = 0; i < count; ++i) {
     uint32_t npts = 0;
     if ((rc = validate_args(args[i], &transcripts[i], &npts))) return rc;
-    insts[i].reset(make_instance(args[i].table_field, args[i].chal_field, args[i].num_tables,
-                                 npts, args[i].mode == ZS_MODE_EQ));
-    if (!insts[i]) return fail(ZS_EINVAL, "unsupported field / degree combination");
-    if ((rc = insts[i]->setup(ctx, args[i], transcripts[i]))) return rc;
+    const bool eq = args[i].mode == ZS_MODE_EQ;
+    auto key = std::make_tuple(args[i].table_field, args[i].chal_field, args[i].num_tables, npts, eq);
+    auto& g = groups[key];
+    if (!g) {
+      g.reset(make_batch_group(args[i].table_field, args[i].chal_field, args[i].num_tables, npts, eq));
+      if (!g) return fail(ZS_EINVAL, "unsupported field / degree combination");
+    }
+    g->add(i, args[i], transcripts[i]);
   }
-  for (size_t i = 0; i < count; ++i) {
-    if ((rc = insts[i]->enqueue(0, args[i].rounds, ctx->stream))) return rc;
-    if ((rc = insts[i]->download(ctx->stream))) return rc;
+  uint32_t steps = 0;
+  for (auto& kv : groups) {
+    if ((rc = kv.second->setup(ctx, ctx->stream))) return rc;
+    steps = std::max(steps, kv.second->max_rounds());
   }
+  for (uint32_t st = 0; st < steps; ++st)
+    for (auto& kv : groups)
+      if ((rc = kv.second->enqueue_step(st, ctx->stream))) return rc;
+  for (auto& kv : groups)
+    if ((rc = kv.second->download(ctx->stream))) return rc;
   ZS_HIP(hipStreamSynchronize(ctx->stream));
-  for (size_t i = 0; i < count; ++i)
-    if ((rc = insts[i]->collect(&outs[i], &transcripts[i]))) return rc;
+  for (auto& kv : groups)
+    if ((rc = kv.second->collect(outs, transcripts))) return rc;
   return ZS_OK;
 }
 

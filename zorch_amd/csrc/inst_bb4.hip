@@ -1,8 +1,12 @@
-// Proof instances with BB4 tables and BB4 challenges.
+// Proof instances and batch groups with BB4 tables and BB4 challenges.
+#include "batch.h"
 #include "instance.h"
 
 namespace zs {
 Instance* make_instance_bb4(uint32_t k, uint32_t npts, bool eq) {
   return make_typed<BB4, BB4>(k, npts, eq);
+}
+BatchGroup* make_batch_group_bb4(uint32_t k, uint32_t npts, bool eq) {
+  return make_typed_batch<BB4, BB4>(k, npts, eq);
 }
 }  // namespace zs

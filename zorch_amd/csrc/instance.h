@@ -31,6 +31,46 @@ static_assert(TUNE_MAX_GRID == (uint32_t)MAXG, "grid limit");
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Interpolation constants of a control block over the nodes {0..NPTS-1}.
+template <class FC, int K, int NPTS>
+void init_ctl_consts(Ctl<FC>& c) {
+  c.one = FC::one();
+  for (int tt = 0; tt < NPTS; ++tt) {
+    int64_t den = 1;
+    for (int j = 0; j < NPTS; ++j)
+      if (j != tt) den *= (int64_t)(tt - j);
+    c.lag_inv[tt] = FC::inv(FC::from_int(den));
+    c.nodes[tt] = FC::from_int(tt);
+  }
+  int64_t binom = 1;  // C(K+1, i)
+  for (int i = 0; i <= K; ++i) {
+    c.ext_coef[i] = FC::from_int(((K - i) & 1) ? -binom : binom);
+    binom = binom * (K + 1 - i) / (i + 1);
+  }
+}
+
+// The per-proof part of a control block: transcript, claim, running state.
+template <class FC>
+void set_ctl_proof(Ctl<FC>& c, const zs_transcript& t, const void* claim) {
+  memcpy(&c.sha, &t.sha, sizeof(Sha256State));
+  c.item_bytes = t.item_bytes;
+  c.limbs = (uint32_t)(sizeof(typename FC::T) / t.item_bytes);
+  c.counter = 0;
+  c.claim = load_host_el<FC>(claim, 0);
+  c.r = FC::zero();
+  c.eq_prev = FC::one();
+}
+
+// Eq point and the coefficients of l_i(t) = eq_prev ((1 - w_i) + t (2 w_i - 1)).
+template <class FC>
+void fill_eq_point(typename FC::T* ep, typename FC::T* ab, const void* eq_point, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) {
+    ep[i] = load_host_el<FC>(eq_point, i);
+    ab[2 * i] = FC::sub(FC::add(ep[i], ep[i]), FC::one());  // 2 w_i - 1
+    ab[2 * i + 1] = FC::sub(FC::one(), ep[i]);              // 1 - w_i
+  }
+}
+
 template <class FT, class FC, int K, int NPTS, bool EQ>
 class InstanceImpl final : public Instance {
   using TT = typename FT::T;
@@ -104,35 +144,9 @@ class InstanceImpl final : public Instance {
     uint8_t* h = (uint8_t*)host_;
     memset(h, 0, io_bytes_);
     Ctl<FC>& c = *(Ctl<FC>*)(h + o_ctl_);
-    memcpy(&c.sha, &t.sha, sizeof(Sha256State));
-    c.item_bytes = t.item_bytes;
-    c.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
-    c.counter = 0;
-    c.claim = load_host_el<FC>(a.claim, 0);
-    c.r = FC::zero();
-    c.eq_prev = FC::one();
-    c.one = FC::one();
-    for (int tt = 0; tt < NPTS; ++tt) {
-      int64_t den = 1;
-      for (int j = 0; j < NPTS; ++j)
-        if (j != tt) den *= (int64_t)(tt - j);
-      c.lag_inv[tt] = FC::inv(FC::from_int(den));
-      c.nodes[tt] = FC::from_int(tt);
-    }
-    int64_t binom = 1;  // C(K+1, i)
-    for (int i = 0; i <= K; ++i) {
-      c.ext_coef[i] = FC::from_int(((K - i) & 1) ? -binom : binom);
-      binom = binom * (K + 1 - i) / (i + 1);
-    }
-    if (EQ) {
-      TC* ep = (TC*)(h + o_eqp);
-      TC* ab = (TC*)(h + o_eqab);
-      for (uint32_t i = 0; i < n_; ++i) {
-        ep[i] = load_host_el<FC>(a.eq_point, i);
-        ab[2 * i] = FC::sub(FC::add(ep[i], ep[i]), FC::one());  // 2 w_i - 1
-        ab[2 * i + 1] = FC::sub(FC::one(), ep[i]);              // 1 - w_i
-      }
-    }
+    init_ctl_consts<FC, K, NPTS>(c);
+    set_ctl_proof<FC>(c, t, a.claim);
+    if (EQ) fill_eq_point<FC>((TC*)(h + o_eqp), (TC*)(h + o_eqab), a.eq_point, n_);
     ZS_HIP(hipMemcpyAsync(b, h, io_bytes_, hipMemcpyHostToDevice, s));
     if (EQ) {
       const size_t entries = fam_offset(l_half) + fam_offset(n_ - l_half);

@@ -681,49 +681,25 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(St
                                        gridDim.x, sh);
 }
 
-// One round of many proofs in one launch: item i owns blocks
-// [block_off, block_off + blocks); `owner` maps each block to its item.
-template <class FT, class FC>
-struct BatchItem {
-  StageArgs<FT, FC> a;
-  uint32_t round;
-  uint32_t blocks;
-  uint32_t block_off;
-  uint32_t L;
+// LDS of the single-block tail loop.
+template <class FC, int NA, int THREADS>
+struct TailLds {
+  typename FC::T lds[(THREADS / 64) * NA];
+  typename FC::T out[NA];
+  FinishLds<FC> sm;
 };
 
-template <class FT, class FC, int K, bool EQ>
-__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(const BatchItem<FT, FC>* items,
-                                                             const uint32_t* owner) {
-  __shared__ RoundLds<FC, K + 1> sh;
-  const BatchItem<FT, FC>& it = items[owner[blockIdx.x]];
-  const uint32_t blk = blockIdx.x - it.block_off;
-  const bool need_claim = it.round + 1 == it.a.n;
-  if (it.round == 0) {
-    round_block<FT, FT, FC, K, EQ, false>(it.a, it.a.in, it.a.in_stride, it.L, 0, need_claim, blk,
-                                          it.blocks, sh);
-  } else if (it.round == 1) {
-    round_block<FT, FT, FC, K, EQ, true>(it.a, it.a.in, it.a.in_stride, it.L, 1, need_claim, blk,
-                                         it.blocks, sh);
-  } else {
-    round_block<FT, FC, FC, K, EQ, true>(it.a, it.a.work, it.a.work_stride, it.L, it.round,
-                                         need_claim, blk, it.blocks, sh);
-  }
-}
-
-// Single-block kernel for the small tail rounds [first, last): each round is
-// a block-stride pass, a block reduction and the finisher, separated by
-// barriers. Global writes of one round are visible to the block in the next;
-// the transcript, challenge and eq mass stay with wave 0 until the end.
-template <class FS, class FC, int K, bool EQ>
-__global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a, uint32_t first,
-                                                            uint32_t last) {
+// The small tail rounds [first, last) of one proof in one block of THREADS
+// threads: each round is a block-stride pass, a block reduction and the
+// finisher, separated by barriers. Global writes of one round are visible to
+// the block in the next; the transcript, challenge and eq mass stay in LDS
+// until the end.
+template <class FS, class FC, int K, bool EQ, int THREADS>
+__device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t first,
+                                            uint32_t last, TailLds<FC, K + 1, THREADS>& sh) {
   using T = typename FC::T;
   constexpr int NA = K + 1;
-  __shared__ T lds[(TAIL_THREADS / 64) * NA];
-  __shared__ T out[NA];
-  __shared__ FinishLds<FC> sm;
-  if (threadIdx.x < 64) fin_load(sm, a.ctl);
+  if (threadIdx.x < 64) fin_load(sh.sm, a.ctl);
   __syncthreads();
   for (uint32_t round = first; round < last; ++round) {
     const uint32_t L = 1u << (a.n - round);
@@ -735,28 +711,95 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
     uint32_t nxr_log = 0;
     if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
     ChalMul<FC> cm;
-    if (round > 0) cm.init(sm.r);
+    if (round > 0) cm.init(sh.sm.r);
     if (threadIdx.x == 0) trace_mark(a.trace, round, 1);
     if (round == 0) {
       pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
-                                            wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
+                                            wr, nxr_log, threadIdx.x, THREADS, acc);
     } else if (round == 1) {
       pass_accumulate<FS, FC, K, EQ, true>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
-                                           wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
+                                           wr, nxr_log, threadIdx.x, THREADS, acc);
     } else {
       pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride, L, cm,
-                                           wl, wr, nxr_log, threadIdx.x, TAIL_THREADS, acc);
+                                           wl, wr, nxr_log, threadIdx.x, THREADS, acc);
     }
     if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
-    block_reduce<FC, NA, TAIL_THREADS>(acc, lds, out);
+    block_reduce<FC, NA, THREADS>(acc, sh.lds, sh.out);
     if (threadIdx.x == 0) trace_mark(a.trace, round, 3);
     if (threadIdx.x < 64) {
-      finish_round<FS, FC, K, EQ>(a, round, round + 1 == last, out, sm, a.work, a.in,
+      finish_round<FS, FC, K, EQ>(a, round, round + 1 == last, sh.out, sh.sm, a.work, a.in,
                                   round == 0 ? a.in_stride : a.work_stride, round == 0);
     }
     __syncthreads();
   }
-  if (threadIdx.x < 64) fin_store(sm, a.ctl);
+  if (threadIdx.x < 64) fin_store(sh.sm, a.ctl);
+}
+
+// Single-block kernel for the tail rounds of one proof.
+template <class FS, class FC, int K, bool EQ>
+__global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a, uint32_t first,
+                                                            uint32_t last) {
+  __shared__ TailLds<FC, K + 1, TAIL_THREADS> sh;
+  tail_rounds<FS, FC, K, EQ, TAIL_THREADS>(a, first, last, sh);
+}
+
+// ------------------------------------------------------------ batched steps
+// One step of a batch: round `round` of every proof still running. Item i
+// owns blocks [block_off, block_off + blocks) of the launch; a `tail` item is
+// one block that runs all of its proof's remaining rounds. Items are sorted
+// by block_off.
+struct BatchItem {
+  uint32_t inst;       // index into the batch's StageArgs array
+  uint32_t blocks;     // power of two
+  uint32_t block_off;
+  uint32_t tail;       // 1: rounds [round, n) in this block
+};
+
+// Item of this block: the number of items whose block_off <= blockIdx.x,
+// minus one, counted by the whole block in chunks of PASS_THREADS items.
+__device__ __forceinline__ uint32_t find_item(const BatchItem* items, uint32_t nitems) {
+  uint32_t below = 0;
+  for (uint32_t base = 0; base < nitems; base += PASS_THREADS) {
+    const uint32_t i = base + threadIdx.x;
+    below += __syncthreads_count(i < nitems && items[i].block_off <= blockIdx.x);
+  }
+  return below - 1;
+}
+
+template <class FT, class FC, int NA>
+union BatchLds {
+  RoundLds<FC, NA> round;
+  TailLds<FC, NA, PASS_THREADS> tail;
+};
+
+// RT: 0 = round 0 (input tables, no fold), 1 = round 1 (fold of the input
+// tables), 2 = later rounds (fold in the challenge field). Every proof of one
+// step is at the same round, so RT is uniform over the launch.
+template <class FT, class FC, int K, bool EQ, int RT>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
+    const StageArgs<FT, FC>* __restrict__ args, const BatchItem* __restrict__ items,
+    uint32_t nitems, uint32_t round) {
+  __shared__ BatchLds<FT, FC, K + 1> sh;
+  const uint32_t idx = find_item(items, nitems);
+  const BatchItem it = items[idx];
+  const StageArgs<FT, FC> a = args[it.inst];
+  if (it.tail) {
+    tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, a.n, sh.tail);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - it.block_off;
+  const uint32_t L = 1u << (a.n - round);
+  const bool need_claim = round + 1 == a.n;
+  if constexpr (RT == 0) {
+    round_block<FT, FT, FC, K, EQ, false>(a, a.in, a.in_stride, L, round, need_claim, blk,
+                                          it.blocks, sh.round);
+  } else if constexpr (RT == 1) {
+    round_block<FT, FT, FC, K, EQ, true>(a, a.in, a.in_stride, L, round, need_claim, blk,
+                                         it.blocks, sh.round);
+  } else {
+    round_block<FT, FC, FC, K, EQ, true>(a, a.work, a.work_stride, L, round, need_claim, blk,
+                                         it.blocks, sh.round);
+  }
 }
 
 // Suffix eq families of the two halves of the eq point (compute_eq_evaluations,
@@ -766,12 +809,11 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
 // entry is its product over coordinates, prod_i (bit_i ? c_i : 1 - c_i),
 // computed independently by one thread (the same field value, exactly).
 template <class FC>
-__global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v, uint32_t h_l,
-                                                        typename FC::T* fam_l, uint32_t h_r,
-                                                        typename FC::T* fam_r) {
+__device__ __forceinline__ void eq_family_entry(const typename FC::T* v, uint32_t h_l,
+                                                typename FC::T* fam_l, uint32_t h_r,
+                                                typename FC::T* fam_r, size_t g) {
   using T = typename FC::T;
   const size_t n_l = fam_offset(h_l), n_r = fam_offset(h_r);
-  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_l + n_r) return;
   const bool left = g < n_l;
   const size_t idx = left ? g : g - n_l;
@@ -788,6 +830,38 @@ __global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v,
     acc = FC::mul(acc, bit ? c : FC::sub(one, c));
   }
   fam[idx] = acc;
+}
+
+template <class FC>
+__global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v, uint32_t h_l,
+                                                        typename FC::T* fam_l, uint32_t h_r,
+                                                        typename FC::T* fam_r) {
+  eq_family_entry<FC>(v, h_l, fam_l, h_r, fam_r, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The eq families of many proofs in one launch: job i owns blocks
+// [block_off, next block_off), 256 entries per block.
+template <class FC>
+struct EqFamJob {
+  const typename FC::T* v;
+  typename FC::T* fam_l;
+  typename FC::T* fam_r;
+  uint32_t h_l, h_r;
+  uint32_t block_off;
+  uint32_t pad;
+};
+
+template <class FC>
+__global__ __launch_bounds__(256) void eq_family_batch_kernel(const EqFamJob<FC>* jobs,
+                                                              uint32_t njobs) {
+  uint32_t lo = 0, hi = njobs;  // last job with block_off <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].block_off <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const EqFamJob<FC> j = jobs[lo];
+  eq_family_entry<FC>(j.v, j.h_l, j.fam_l, j.h_r, j.fam_r,
+                      (size_t)(blockIdx.x - j.block_off) * 256 + threadIdx.x);
 }
 
 }  // namespace zs

@@ -74,7 +74,7 @@ struct KernelRecord {
   uint64_t bytes;
   hipEvent_t start, stop;
 };
-enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3 };
+enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4 };
 
 }  // namespace zs
 
@@ -128,9 +128,34 @@ class Instance {
   uint64_t device_bytes = 0;
 };
 
+// Many proofs of one kind (field pair, factor count, mode) proved together:
+// each step launches one round of every member still running.
+class BatchGroup {
+ public:
+  virtual ~BatchGroup() = default;
+  // Register member `index` (its args and transcript are read in setup).
+  virtual void add(size_t index, const zs_prove_args& a, const zs_transcript& t) = 0;
+  // Allocate, build the step schedule, upload control blocks and inputs.
+  virtual int setup(zs_ctx* ctx, hipStream_t stream) = 0;
+  virtual uint32_t max_rounds() const = 0;
+  // Launch step `step` (round `step` of every member still running).
+  virtual int enqueue_step(uint32_t step, hipStream_t stream) = 0;
+  virtual int download(hipStream_t stream) = 0;
+  // After the stream has synchronised: outputs of every member, by index.
+  virtual int collect(zs_proof* outs, zs_transcript* ts) = 0;
+  virtual void free_buffers() = 0;
+  uint64_t device_bytes = 0;
+};
+
 // Factory: returns nullptr for an unsupported combination.
 Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
                         bool eq);
+BatchGroup* make_batch_group(uint32_t table_field, uint32_t chal_field, uint32_t k,
+                             uint32_t npts, bool eq);
+BatchGroup* make_batch_group_bb(uint32_t k, uint32_t npts, bool eq);
+BatchGroup* make_batch_group_bb_bb4(uint32_t k, uint32_t npts, bool eq);
+BatchGroup* make_batch_group_bb4(uint32_t k, uint32_t npts, bool eq);
+BatchGroup* make_batch_group_gl(uint32_t k, uint32_t npts, bool eq);
 // Per-field-combination factories (one translation unit each).
 Instance* make_instance_bb(uint32_t k, uint32_t npts, bool eq);
 Instance* make_instance_bb_bb4(uint32_t k, uint32_t npts, bool eq);
