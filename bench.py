@@ -193,6 +193,133 @@ def bench_batch(args) -> dict:
     }
 
 
+def bench_serve(args) -> dict:
+    """Served proving: Poisson arrivals (2000/s, n ~ U{14..24}, degree-3
+    zerocheck over BabyBear^4) through the continuous-batching server, against
+    serving the same trace one proof per call, first come first served."""
+    from zorch_amd import _lib
+    from zorch_amd import workloads as W
+    from zorch_amd.runtime import DeviceTables, default_context
+    from zorch_amd.serve import ProofServer
+    from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+    from zorch_amd.sumcheck._device import make_job, prove_stage
+
+    spec = W.serve_spec(rate=args.rate, duration=args.duration)
+    ctx = default_context()
+    lib = _lib.lib()
+    # One device-resident input per size; every request has its own transcript.
+    base = {n: W.zerocheck_bb4(n, (2000 + n, 2100 + n), 2200 + n, 2300 + n)
+            for n in sorted(set(spec.sizes))}
+    dev = {n: DeviceTables.from_host(base[n].tables, ctx) for n in base}
+    eq = _lib.ZS_MODE_EQ
+
+    def transcript(i):
+        inst = base[spec.sizes[i]]
+        return Sha256FieldTranscript.new(W.label(1_000_000 + i), inst.transcript_field)
+
+    def submit(srv, i):
+        inst = base[spec.sizes[i]]
+        return srv.submit(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, transcript(i),
+                          inst.z)
+
+    def pct(xs, q):
+        return float(np.percentile(np.asarray(xs) * 1e3, q)) if xs else None
+
+    # Open loop through the server over the whole trace.
+    N = len(spec.arrivals)
+    lat, sample = [], {}
+    with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+        ids = {}
+        i = done = 0
+        t0 = time.perf_counter()
+        while done < N:
+            now = time.perf_counter() - t0
+            busy = False
+            while i < N and spec.arrivals[i] <= now:
+                ids[submit(srv, i)] = i
+                i += 1
+                busy = True
+            for rid in srv.completed():
+                res, l = srv.poll(rid)
+                lat.append(l)
+                k = ids.pop(rid)
+                if k % max(N // 16, 1) == 0:
+                    sample[k] = res.msgs.words.copy()
+                done += 1
+                busy = True
+            if not busy:
+                time.sleep(0.0002)
+        served_s = time.perf_counter() - t0
+        sst = srv.stats()
+    # The same requests alone give the same proofs.
+    identical = True
+    for k, msgs in sample.items():
+        inst = base[spec.sizes[k]]
+        alone = prove_stage(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, transcript(k),
+                            inst.z, ctx=ctx)
+        identical &= bool((alone.msgs.words == msgs).all())
+        identical &= _verify_proof(inst, msgs, transcript(k))
+
+    # One proof per call, first come first served, over the first part of the trace.
+    Np = int(np.searchsorted(spec.arrivals, args.plain_duration))
+    jobs = {n: make_job(dev[n], base[n].claim, base[n].chal, eq, base[n].degree, base[n].z)
+            for n in base}
+    plain_lat = []
+    t0 = time.perf_counter()
+    for j in range(Np):
+        now = time.perf_counter() - t0
+        if now < spec.arrivals[j]:
+            time.sleep(spec.arrivals[j] - now)
+        job = jobs[spec.sizes[j]]
+        st = transcript(j).state.copy()
+        _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
+                                ctypes.byref(job.proof)))
+        plain_lat.append(time.perf_counter() - t0 - spec.arrivals[j])
+    plain_s = time.perf_counter() - t0
+
+    # Capacity: the first C requests all at once, served versus one per call.
+    C = min(args.capacity_requests, N)
+    with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+        t0 = time.perf_counter()
+        cids = [submit(srv, j) for j in range(C)]
+        for rid in cids:
+            srv.wait(rid)
+        cap_served = C / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for j in range(C):
+        job = jobs[spec.sizes[j]]
+        st = transcript(j).state.copy()
+        _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
+                                ctypes.byref(job.proof)))
+    cap_plain = C / (time.perf_counter() - t0)
+    elems = sum(2 * (1 << n) for n in spec.sizes)
+    for d in dev.values():
+        d.free()
+    return {
+        "metric": "served proofs/s and p50/p99 latency (Poisson arrivals, n ~ U{14..24})",
+        "value": N / served_s, "unit": "proofs/s", "n_gpus": 1,
+        "steps": N, "warmup": 0, "ms_per_step": served_s / N * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 (BabyBear Montgomery / BabyBear^4)",
+        "data": "synthetic (seeded uniform field elements; one input per size, one "
+                "transcript label per request)",
+        "config": {"workload": "serve_poisson", "rate_per_s": args.rate,
+                   "duration_s": args.duration, "requests": N, "sizes": "U{14..24} seed 5",
+                   "kind": "zerocheck eq(z,x) A B over babybearx4_mont",
+                   "hbm_budget_gb": args.hbm_budget_gb, "parallelism": "single GPU"},
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": pct(lat, 100),
+        "table_elems_per_s": elems / served_s,
+        "server": {k: sst[k] for k in ("steps", "peak_active", "peak_resident_bytes",
+                                       "busy_s", "failed")},
+        "plain": {"requests": Np, "window_s": args.plain_duration,
+                  "proofs_per_s": Np / plain_s, "p50_ms": pct(plain_lat, 50),
+                  "p99_ms": pct(plain_lat, 99)},
+        "capacity": {"requests": C, "served_proofs_per_s": cap_served,
+                     "plain_proofs_per_s": cap_plain, "speedup": cap_served / cap_plain},
+        "identical_to_alone": bool(identical), "sampled": len(sample),
+    }
+
+
 def cpu_baseline(inst, threads: int) -> dict:
     """The C oracle (the reference algorithm restated) on the host cores."""
     from tests.oracle_glue import run_c
@@ -219,6 +346,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--plain-duration", type=float, default=10.0)
+    ap.add_argument("--capacity-requests", type=int, default=2000)
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -231,6 +363,9 @@ def main() -> int:
         return 0
     if args.workload == "batch_512":
         print(json.dumps(bench_batch(args)), flush=True)
+        return 0
+    if args.workload == "serve_poisson":
+        print(json.dumps(bench_serve(args)), flush=True)
         return 0
 
     from zorch_amd import _lib

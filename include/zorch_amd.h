@@ -211,6 +211,48 @@ int zs_verify_round(uint32_t chal_field, zs_transcript* transcript, const void* 
                     const void* msg, uint32_t npts, uint32_t degree, void* chal_out,
                     void* reduced_out, int* ok);
 
+/* ---- proof server (continuous batching) ----------------------------------
+ * No counterpart in the reference, which proves one stage per call; this is
+ * the serving loop for many independent requests on one GPU. A server takes
+ * over a context (do not use the context elsewhere until zs_server_destroy).
+ * Its scheduler thread admits submitted requests in arrival order against an
+ * HBM budget (resident state per request: control block, outputs, eq
+ * families, folded tables, and the input tables when they are on the host),
+ * then repeatedly runs one step: the next round of every admitted request,
+ * one batched launch per request kind, so that requests which arrive while
+ * others are mid-proof join at the next step. A request whose remaining
+ * rounds are small finishes them all within one step. Results are those of
+ * zs_prove for the same arguments and transcript. Submit, poll, wait and
+ * completed may be called from any thread. */
+typedef struct zs_server zs_server;
+typedef struct zs_server_stats {
+  uint64_t submitted, admitted, completed, failed, steps;
+  uint64_t queued, active, resident_bytes, peak_resident_bytes, peak_active;
+  uint64_t budget_bytes;
+  double busy_s; /* scheduler time inside steps */
+} zs_server_stats;
+/* hbm_budget_bytes = 0: the context's budget. */
+int zs_server_create(zs_ctx* ctx, uint64_t hbm_budget_bytes, zs_server** out);
+/* args->tables must stay valid until the request completes; the claim and
+ * eq point are copied; the transcript is read. A request larger than the
+ * whole budget completes with ZS_EBUDGET. */
+int zs_server_submit(zs_server* srv, const zs_prove_args* args, const zs_transcript* transcript,
+                     uint64_t* id);
+/* done = 0: still running. done = 1: `out` filled as by zs_prove, the
+ * transcript state advanced, latency_s = submit to completion on the
+ * server's clock; the request is forgotten (its error, if any, is returned). */
+int zs_server_poll(zs_server* srv, uint64_t id, zs_proof* out, zs_transcript* transcript,
+                   double* latency_s, int* done);
+/* As poll, blocking up to timeout_s seconds. */
+int zs_server_wait(zs_server* srv, uint64_t id, zs_proof* out, zs_transcript* transcript,
+                   double* latency_s, double timeout_s, int* done);
+/* Ids of requests finished since the last call (at most cap). */
+int zs_server_completed(zs_server* srv, uint64_t* ids, size_t cap, size_t* count);
+int zs_server_stats_read(zs_server* srv, zs_server_stats* out);
+/* Stops the scheduler after the step in flight; unfinished requests are
+ * dropped. The context is free for other use afterwards. */
+void zs_server_destroy(zs_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,121 @@
+"""GPU parity of the continuous-batching proof server.
+
+Requests submitted at different times (so that later ones join while earlier
+ones are mid-proof) must each equal the proof of the same request made alone,
+bit for bit; admission respects the HBM budget; a request larger than the
+whole budget fails with MemoryError instead of blocking the queue.
+"""
+
+from __future__ import annotations
+
+import time
+
+import pytest
+
+from zorch_amd import _lib
+from zorch_amd import workloads as W
+from zorch_amd.runtime import Context, DeviceTables
+from zorch_amd.serve import ProofServer
+from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+from zorch_amd.sumcheck._device import prove_stage
+
+pytestmark = pytest.mark.gpu
+
+
+def _mode(inst):
+    return _lib.ZS_MODE_EQ if inst.mode == "zerocheck" else _lib.ZS_MODE_PRODUCT
+
+
+def _t0(inst):
+    return Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+
+
+def _submit(srv, inst, tables=None):
+    return srv.submit(inst.tables if tables is None else tables, inst.claim, inst.chal,
+                      _mode(inst), inst.degree, _t0(inst), inst.z)
+
+
+def _alone(inst, ctx):
+    return prove_stage(inst.tables, inst.claim, inst.chal, _mode(inst), inst.degree, _t0(inst),
+                       inst.z, ctx=ctx)
+
+
+def _same(a, b):
+    assert a.msgs.words.tolist() == b.msgs.words.tolist()
+    assert a.point.words.tolist() == b.point.words.tolist()
+    assert a.value.words.tolist() == b.value.words.tolist()
+    assert a.final.words.tolist() == b.final.words.tolist()
+    assert a.transcript.stream_state() == b.transcript.stream_state()
+
+
+def _requests():
+    insts = []
+    for j, n in enumerate([16, 3, 12, 14, 1, 13, 9, 15, 11, 18]):
+        insts.append(W.zerocheck_bb4(n, (1000 + j, 1100 + j), 1200 + j, 1300 + j))
+    insts.append(W.product_bb_ext(13, (1401, 1402), 1403))
+    insts.append(W.product_bb(12, (1404, 1405, 1406), 1407))
+    insts.append(W.product_gl(14, (1411, 1412), 1413))
+    return insts
+
+
+def test_served_results_equal_alone():
+    ctx = Context(0)
+    insts = _requests()
+    ref = [_alone(i, ctx) for i in insts]
+    with ProofServer(ctx) as srv:
+        ids = []
+        for k, inst in enumerate(insts):
+            ids.append(_submit(srv, inst))
+            if k % 3 == 2:
+                time.sleep(0.002)  # later requests join a running batch
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for (res, lat), r in zip(got, ref):
+        _same(res, r)
+        assert lat > 0
+    assert st["completed"] == len(insts) and st["failed"] == 0
+    assert st["steps"] >= 1 and st["resident_bytes"] == 0
+    ctx.close()
+
+
+def test_budget_admission_and_oversized_request():
+    ctx = Context(0)
+    insts = [W.zerocheck_bb4(14, (1500 + j, 1600 + j), 1700 + j, 1800 + j) for j in range(6)]
+    dev = [DeviceTables.from_host(i.tables, ctx) for i in insts]
+    ref = [_alone(i, ctx) for i in insts]
+    # Room for two n=14 requests at a time (device-resident inputs: about
+    # 280 KB each, mostly the folded tables).
+    budget = 600_000
+    with ProofServer(ctx, budget) as srv:
+        ids = [_submit(srv, i, d) for i, d in zip(insts, dev)]
+        big = W.zerocheck_bb4(17, (1901, 1902), 1903, 1904)
+        big_id = _submit(srv, big)
+        got = [srv.wait(i) for i in ids]
+        with pytest.raises(MemoryError):
+            srv.wait(big_id)
+        st = srv.stats()
+    for (res, _), r in zip(got, ref):
+        _same(res, r)
+    assert st["peak_resident_bytes"] <= budget
+    assert st["completed"] == len(insts) and st["failed"] == 1
+    for d in dev:
+        d.free()
+    ctx.close()
+
+
+def test_poll_and_completed_ids():
+    ctx = Context(0)
+    insts = _requests()[:5]
+    with ProofServer(ctx) as srv:
+        ids = [_submit(srv, i) for i in insts]
+        seen = set()
+        deadline = time.time() + 60
+        while len(seen) < len(ids) and time.time() < deadline:
+            for rid in srv.completed():
+                assert srv.poll(rid) is not None
+                seen.add(rid)
+            time.sleep(0.001)
+        assert seen == set(ids)
+        with pytest.raises(ValueError):
+            srv.poll(ids[0])  # already taken
+    ctx.close()

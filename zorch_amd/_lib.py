@@ -88,6 +88,13 @@ class TuneC(ctypes.Structure):
     ]
 
 
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "admitted", "completed", "failed", "steps", "queued", "active",
+        "resident_bytes", "peak_resident_bytes", "peak_active", "budget_bytes")] + [
+        ("busy_s", ctypes.c_double)]
+
+
 assert ctypes.sizeof(Sha256StateC) == 112
 assert ctypes.sizeof(TranscriptC) == 128
 
@@ -132,6 +139,17 @@ SIGNATURES = {
     "zs_state_destroy": (None, [_P]),
     "zs_verify_round": (_INT, [_U32, ctypes.POINTER(TranscriptC), _P, _P, _U32, _U32, _P, _P,
                                ctypes.POINTER(_INT)]),
+    "zs_server_create": (_INT, [_P, _U64, ctypes.POINTER(_P)]),
+    "zs_server_submit": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
+                                ctypes.POINTER(_U64)]),
+    "zs_server_poll": (_INT, [_P, _U64, ctypes.POINTER(ProofC), ctypes.POINTER(TranscriptC),
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_INT)]),
+    "zs_server_wait": (_INT, [_P, _U64, ctypes.POINTER(ProofC), ctypes.POINTER(TranscriptC),
+                              ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                              ctypes.POINTER(_INT)]),
+    "zs_server_completed": (_INT, [_P, ctypes.POINTER(_U64), _SZ, ctypes.POINTER(_SZ)]),
+    "zs_server_stats_read": (_INT, [_P, ctypes.POINTER(ServerStatsC)]),
+    "zs_server_destroy": (None, [_P]),
 }
 
 _lib = None

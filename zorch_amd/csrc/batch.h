@@ -197,11 +197,11 @@ class BatchGroupImpl final : public BatchGroup {
       (void)hipEventRecord(ev, s);
     }
     if (st == 0) {
-      batch_kernel<FT, FC, K, EQ, 0><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems, st);
+      batch_kernel<FT, FC, K, EQ, 0><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
     } else if (st == 1) {
-      batch_kernel<FT, FC, K, EQ, 1><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems, st);
+      batch_kernel<FT, FC, K, EQ, 1><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
     } else {
-      batch_kernel<FT, FC, K, EQ, 2><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems, st);
+      batch_kernel<FT, FC, K, EQ, 2><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
     }
     ZS_HIP(hipGetLastError());
     if (ev) {
@@ -254,7 +254,7 @@ class BatchGroupImpl final : public BatchGroup {
         if (done[i] || n <= st) continue;
         const size_t pairs = (size_t)1 << (n - st - 1);
         if (pairs <= tune_.tail_pairs) {
-          tails.push_back(BatchItem{(uint32_t)i, 1, 0, 1});
+          tails.push_back(BatchItem{(uint32_t)i, 1, 0, st | TAIL_FLAG});
           done[i] = 1;
           for (uint32_t r = st; r < n; ++r) S.bytes += pass_bytes(n, r);
           continue;
@@ -264,7 +264,7 @@ class BatchGroupImpl final : public BatchGroup {
         if (grid > maxg) grid = maxg;
         if (grid < 1) grid = 1;
         while (grid & (grid - 1)) grid &= grid - 1;
-        passes.push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, 0});
+        passes.push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, st});
         S.bytes += pass_bytes(n, st);
       }
       std::stable_sort(passes.begin(), passes.end(),

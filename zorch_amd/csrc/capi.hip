@@ -152,6 +152,14 @@ BatchGroup* make_batch_group(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts
   return nullptr;
 }
 
+ServeKind* make_serve_kind(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_serve_kind_bb(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_serve_kind_bb_bb4(k, npts, eq);
+  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_serve_kind_bb4(k, npts, eq);
+  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_serve_kind_gl(k, npts, eq);
+  return nullptr;
+}
+
 Instance* make_instance(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
   if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_instance_bb(k, npts, eq);
   if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_instance_bb_bb4(k, npts, eq);

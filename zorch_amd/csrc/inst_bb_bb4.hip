@@ -1,6 +1,7 @@
-// Proof instances and batch groups with BB tables and BB4 challenges.
+// Proof instances, batch groups and server kinds with BB tables and BB4 challenges.
 #include "batch.h"
 #include "instance.h"
+#include "serve.h"
 
 namespace zs {
 Instance* make_instance_bb_bb4(uint32_t k, uint32_t npts, bool eq) {
@@ -8,5 +9,8 @@ Instance* make_instance_bb_bb4(uint32_t k, uint32_t npts, bool eq) {
 }
 BatchGroup* make_batch_group_bb_bb4(uint32_t k, uint32_t npts, bool eq) {
   return make_typed_batch<BB, BB4>(k, npts, eq);
+}
+ServeKind* make_serve_kind_bb_bb4(uint32_t k, uint32_t npts, bool eq) {
+  return make_typed_serve<BB, BB4>(k, npts, eq);
 }
 }  // namespace zs

@@ -744,16 +744,17 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
 }
 
 // ------------------------------------------------------------ batched steps
-// One step of a batch: round `round` of every proof still running. Item i
-// owns blocks [block_off, block_off + blocks) of the launch; a `tail` item is
-// one block that runs all of its proof's remaining rounds. Items are sorted
-// by block_off.
+// One step over many proofs: each item is one round of one proof. Item i owns
+// blocks [block_off, block_off + blocks) of the launch; a tail item is one
+// block that runs all of its proof's remaining rounds. Items are sorted by
+// block_off.
 struct BatchItem {
-  uint32_t inst;       // index into the batch's StageArgs array
-  uint32_t blocks;     // power of two
+  uint32_t inst;        // index into the step's StageArgs array
+  uint32_t blocks;      // power of two
   uint32_t block_off;
-  uint32_t tail;       // 1: rounds [round, n) in this block
+  uint32_t round_tail;  // round | TAIL_FLAG: rounds [round, n) in this block
 };
+constexpr uint32_t TAIL_FLAG = 0x80000000u;
 
 // Item of this block: the number of items whose block_off <= blockIdx.x,
 // minus one, counted by the whole block in chunks of PASS_THREADS items.
@@ -772,31 +773,34 @@ union BatchLds {
   TailLds<FC, NA, PASS_THREADS> tail;
 };
 
-// RT: 0 = round 0 (input tables, no fold), 1 = round 1 (fold of the input
-// tables), 2 = later rounds (fold in the challenge field). Every proof of one
-// step is at the same round, so RT is uniform over the launch.
+// RT: the round type of every item of the launch: 0 = round 0 (input tables,
+// no fold), 1 = round 1 (fold of the input tables), 2 = later rounds (fold in
+// the challenge field); 3 = mixed, chosen per item (proofs that joined a
+// continuous batch at different steps).
 template <class FT, class FC, int K, bool EQ, int RT>
 __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
     const StageArgs<FT, FC>* __restrict__ args, const BatchItem* __restrict__ items,
-    uint32_t nitems, uint32_t round) {
+    uint32_t nitems) {
   __shared__ BatchLds<FT, FC, K + 1> sh;
   const uint32_t idx = find_item(items, nitems);
   const BatchItem it = items[idx];
   const StageArgs<FT, FC> a = args[it.inst];
-  if (it.tail) {
+  const uint32_t round = it.round_tail & ~TAIL_FLAG;
+  if (it.round_tail & TAIL_FLAG) {
     tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, a.n, sh.tail);
     return;
   }
   const uint32_t blk = blockIdx.x - it.block_off;
   const uint32_t L = 1u << (a.n - round);
   const bool need_claim = round + 1 == a.n;
-  if constexpr (RT == 0) {
+  const int rt = RT == 3 ? (round == 0 ? 0 : round == 1 ? 1 : 2) : RT;
+  if ((RT == 0 || RT == 3) && rt == 0) {
     round_block<FT, FT, FC, K, EQ, false>(a, a.in, a.in_stride, L, round, need_claim, blk,
                                           it.blocks, sh.round);
-  } else if constexpr (RT == 1) {
+  } else if ((RT == 1 || RT == 3) && rt == 1) {
     round_block<FT, FT, FC, K, EQ, true>(a, a.in, a.in_stride, L, round, need_claim, blk,
                                          it.blocks, sh.round);
-  } else {
+  } else if (RT >= 2) {
     round_block<FT, FC, FC, K, EQ, true>(a, a.work, a.work_stride, L, round, need_claim, blk,
                                          it.blocks, sh.round);
   }

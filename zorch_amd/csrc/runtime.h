@@ -147,9 +147,47 @@ class BatchGroup {
   uint64_t device_bytes = 0;
 };
 
+// One admitted request of the proof server (device state lives with its kind).
+class ServeReq {
+ public:
+  virtual ~ServeReq() = default;
+  uint32_t n = 0;
+  uint32_t next = 0;       // next round to run
+  bool finishing = false;  // completes with the step just launched
+  uint64_t bytes = 0;      // device bytes held
+};
+
+// Outputs of a finished request, kept on the host until polled.
+struct ServeResult {
+  std::vector<uint8_t> msgs, point, value, final_tables;
+  zs_sha256_state sha;
+};
+
+// Requests of one kind (field pair, factor count, mode): admission, one
+// batched launch per server step, download and release.
+class ServeKind {
+ public:
+  virtual ~ServeKind() = default;
+  virtual uint64_t bytes_needed(const zs_prove_args& a) const = 0;
+  virtual int admit(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t,
+                    ServeReq** out) = 0;
+  // Launch the next round of every request in `reqs` (one step); advances
+  // their `next` and sets `finishing` on those that complete with it.
+  virtual int step(zs_ctx* ctx, std::vector<ServeReq*>& reqs, hipStream_t stream) = 0;
+  virtual int download(ServeReq* r, hipStream_t stream) = 0;
+  virtual void collect(ServeReq* r, ServeResult* out) = 0;
+  virtual void release(zs_ctx* ctx, ServeReq* r) = 0;
+};
+
 // Factory: returns nullptr for an unsupported combination.
 Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
                         bool eq);
+ServeKind* make_serve_kind(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
+                           bool eq);
+ServeKind* make_serve_kind_bb(uint32_t k, uint32_t npts, bool eq);
+ServeKind* make_serve_kind_bb_bb4(uint32_t k, uint32_t npts, bool eq);
+ServeKind* make_serve_kind_bb4(uint32_t k, uint32_t npts, bool eq);
+ServeKind* make_serve_kind_gl(uint32_t k, uint32_t npts, bool eq);
 BatchGroup* make_batch_group(uint32_t table_field, uint32_t chal_field, uint32_t k,
                              uint32_t npts, bool eq);
 BatchGroup* make_batch_group_bb(uint32_t k, uint32_t npts, bool eq);
