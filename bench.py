@@ -154,6 +154,26 @@ def bench_batch(args) -> dict:
 
     plain()
     plain_msgs = [j.msgs.copy() for j in jobs]
+    default_tune = ctx.get_tune()
+    batched()
+    t = time.perf_counter()
+    batched()
+    default_ms = (time.perf_counter() - t) * 1e3
+    tuned = None
+    if not args.no_autotune:
+        import hashlib
+
+        from zorch_amd import autotune
+
+        def digest():
+            h = hashlib.sha256()
+            for j in jobs:
+                h.update(j.msgs.tobytes())
+            return h.hexdigest()
+
+        best, best_ms, table = autotune.tune(ctx, batched, key=f"batch_{n}", check=digest)
+        tuned = {"best": best, "best_ms": best_ms, "default": default_tune,
+                 "default_ms": default_ms, "candidates": len(table)}
     t = time.perf_counter()
     reps_plain = 2
     for _ in range(reps_plain):
@@ -189,7 +209,7 @@ def bench_batch(args) -> dict:
         "proofs_per_s": n / batch_s, "plain_proofs_per_s": n / plain_s,
         "hbm_gbs": alg / batch_s / 1e9, "algorithmic_bytes": alg,
         "identical_to_plain": bool(identical), "verified": bool(verified),
-        "tune": ctx.get_tune(),
+        "tune": ctx.get_tune(), "autotune": tuned,
     }
 
 
@@ -346,6 +366,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch workload: keep the default launch shapes")
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--plain-duration", type=float, default=10.0)

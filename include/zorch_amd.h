@@ -37,7 +37,8 @@ extern "C" {
 #define ZS_F_GOLDILOCKS 3
 
 /* Transcript kinds. */
-#define ZS_TR_SHA256 1
+#define ZS_TR_SHA256 1 /* Sha256FieldTranscript: counter squeeze SHA256(S || le64(ctr)) */
+#define ZS_TR_BLAKE3 2 /* Blake3FieldTranscript: squeeze = BLAKE3 extendable output of S */
 
 /* Prover modes. */
 #define ZS_MODE_PRODUCT 1 /* StandardRound(ProductSummand(degree)), natural domain */
@@ -54,15 +55,35 @@ typedef struct zs_sha256_state {
   uint64_t total_len;
 } zs_sha256_state;
 
-/* A field transcript value (128 bytes). `field` is the transcript word field
- * and `item_bytes` its storage width. Plain data: copying it forks the
- * transcript, as the reference's immutable transcripts do. */
+/* Streaming BLAKE3 state (hash mode, 632 bytes): the current chunk's
+ * chaining value, the stack of completed subtree chaining values, the current
+ * block (not yet compressed: the last block of a chunk carries CHUNK_END),
+ * the chunk index and counts. Replaces hash-frx's Blake3Stream
+ * (zorch/blake3_field_transcript.py:206-215). */
+typedef struct zs_blake3_state {
+  uint32_t cv[8];
+  uint32_t stack[16][8];
+  uint8_t block[64];
+  uint64_t chunk_counter;
+  uint32_t block_len;
+  uint32_t blocks_done;
+  uint32_t stack_len;
+  uint32_t reserved;
+} zs_blake3_state;
+
+/* A field transcript value (648 bytes). `kind` selects the hash (ZS_TR_*),
+ * `field` is the transcript word field and `item_bytes` its storage width.
+ * Plain data: copying it forks the transcript, as the reference's immutable
+ * transcripts do. */
 typedef struct zs_transcript {
   uint32_t kind;
   uint32_t field;
   uint32_t item_bytes;
   uint32_t reserved;
-  zs_sha256_state sha;
+  union {
+    zs_sha256_state sha;
+    zs_blake3_state blake3;
+  };
 } zs_transcript;
 
 typedef struct zs_prove_args {
@@ -154,18 +175,22 @@ int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out);
 int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* tune);
 
 /* ---- transcript (host) ------------------------------------------------
- * Replaces hash-frx's Sha256State stream under Sha256FieldTranscript
- * (zorch/sha256_field_transcript.py:45-50,100-114,178-181). Framing is
- * composed by the caller exactly as the reference's Python does. */
+ * Replaces hash-frx's streams under Sha256FieldTranscript
+ * (zorch/sha256_field_transcript.py:45-50,100-114,178-181) and
+ * Blake3FieldTranscript (zorch/blake3_field_transcript.py:206-215,293-330).
+ * Framing is composed by the caller exactly as the reference's Python does. */
 int zs_transcript_new(uint32_t kind, uint32_t field, const uint8_t* domain, size_t len,
                       zs_transcript* out);
-/* sha256_stream_absorb (zorch/sha256_field_transcript.py:180,196). */
+/* Stream absorb (zorch/sha256_field_transcript.py:180,196;
+ * zorch/blake3_field_transcript.py:300-301). */
 int zs_transcript_absorb(zs_transcript* t, const uint8_t* data, size_t len);
-/* _squeeze_hop minus its framing absorb: SHA256(S||le64(ctr)) blocks,
- * truncated to nbytes, re-absorbed (zorch/sha256_field_transcript.py:100-114). */
+/* A squeeze minus its framing absorb, re-absorbing what it read: SHA-256,
+ * SHA256(S||le64(ctr)) blocks truncated to nbytes
+ * (zorch/sha256_field_transcript.py:100-114); BLAKE3, the first nbytes of the
+ * extendable output of S (zorch/blake3_field_transcript.py:206-215). */
 int zs_transcript_squeeze(zs_transcript* t, size_t nbytes, uint8_t* out);
-/* SHA256 of the stream so far (the proof-of-work state digest,
- * zorch/sha256_field_transcript.py:277-284). */
+/* The 32-byte hash of the stream so far (the proof-of-work state digest,
+ * zorch/sha256_field_transcript.py:277-284, blake3_field_transcript.py:409). */
 int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]);
 
 /* ---- prover ------------------------------------------------------------

@@ -20,6 +20,7 @@ from __future__ import annotations
 import hashlib
 from dataclasses import dataclass
 
+from oracle import blake3 as _b3
 from oracle import sha256 as _sha
 
 OP_DOMAIN, OP_LABEL, OP_OBSERVE, OP_SQUEEZE, OP_BYTES = 1, 2, 3, 4, 5
@@ -40,7 +41,7 @@ class OracleTranscript:
         return cls(bytes([OP_DOMAIN]) + le64(len(domain)) + bytes(domain), item_bytes)
 
     def absorb(self, payload: bytes) -> "OracleTranscript":
-        return OracleTranscript(self.stream + payload, self.item_bytes)
+        return type(self)(self.stream + payload, self.item_bytes)
 
     def observe(self, value_bytes: bytes) -> "OracleTranscript":
         count = len(value_bytes) // self.item_bytes
@@ -81,3 +82,20 @@ class OracleTranscript:
     def state(self) -> tuple[tuple[int, ...], bytes, int]:
         """Streaming state (midstate h, pending bytes, total length)."""
         return _sha.midstate(self.stream)
+
+
+@dataclass(frozen=True)
+class OracleBlake3Transcript(OracleTranscript):
+    """``Blake3FieldTranscript`` (reference zorch/blake3_field_transcript.py:
+    247-409): the same framing; a squeeze is the first n * s_T bytes of the
+    BLAKE3 extendable output of S (:206-215), re-absorbed."""
+
+    def _squeeze(self, nbytes: int) -> bytes:
+        return _b3.blake3(self.stream, nbytes)
+
+    def state(self) -> tuple:
+        """Streaming state (chunk chaining value, subtree stack, current block,
+        chunk index, blocks compressed in the chunk)."""
+        s = _b3.Blake3Stream().absorb(self.stream)
+        return (tuple(s.cv), tuple(tuple(c) for c in s.stack), bytes(s.block),
+                s.chunk_counter, s.blocks_done)

@@ -27,18 +27,57 @@ RECIPES = {
     # Goldilocks product at reduced n.
     "product_gl_n5": {"kind": "product_gl", "n": 5, "seeds": [1, 2], "label": 0},
     "product_gl_n10": {"kind": "product_gl", "n": 10, "seeds": [1, 2], "label": 0},
+    # The BLAKE3 field transcript; long labels put the stream past one and
+    # past two 1024-byte chunks (subtree merges) before the first round.
+    "product_bb_n4_b3": {"kind": "product_bb", "n": 4, "seeds": [1, 2], "label": 0,
+                         "hash": "blake3"},
+    "product_bb_bb4_n6_b3": {"kind": "product_bb_ext", "n": 6, "seeds": [5, 6], "label": 4,
+                             "hash": "blake3", "label_len": 1000},
+    "zerocheck_bb4_n5_b3": {"kind": "zerocheck", "n": 5, "seeds": [1, 2], "z": 3, "label": 0,
+                            "hash": "blake3"},
+    "zerocheck_bb4_n9_b3": {"kind": "zerocheck", "n": 9, "seeds": [1, 2], "z": 3, "label": 7,
+                            "hash": "blake3", "label_len": 3000},
+    "zerocheck_bb4_t4_n5_b3": {"kind": "zerocheck", "n": 5, "seeds": [1, 2], "z": 3, "label": 0,
+                               "transcript": "babybearx4_mont", "hash": "blake3",
+                               "label_len": 2040},
+    "product_gl_n6_b3": {"kind": "product_gl", "n": 6, "seeds": [1, 2], "label": 0,
+                         "hash": "blake3", "label_len": 5000},
 }
 
 
 def build(recipe: dict) -> W.Instance:
     kind, n, seeds, lab = recipe["kind"], recipe["n"], tuple(recipe["seeds"]), recipe["label"]
     if kind == "product_bb":
-        return W.product_bb(n, seeds, lab)
-    if kind == "product_bb_ext":
-        return W.product_bb_ext(n, seeds, lab)
-    if kind == "product_gl":
-        return W.product_gl(n, seeds, lab)
-    if kind == "zerocheck":
+        inst = W.product_bb(n, seeds, lab)
+    elif kind == "product_bb_ext":
+        inst = W.product_bb_ext(n, seeds, lab)
+    elif kind == "product_gl":
+        inst = W.product_gl(n, seeds, lab)
+    elif kind == "zerocheck":
         tf = babybearx4_mont if recipe.get("transcript") == "babybearx4_mont" else babybear_mont
-        return W.zerocheck_bb4(n, seeds, recipe["z"], lab, transcript_field=tf)
-    raise ValueError(kind)
+        inst = W.zerocheck_bb4(n, seeds, recipe["z"], lab, transcript_field=tf)
+    else:
+        raise ValueError(kind)
+    inst.transcript_kind = recipe.get("hash", "sha256")
+    if "label_len" in recipe:
+        inst.label = W.label(lab, recipe["label_len"])
+    return inst
+
+
+def state_json(state: tuple, kind: str) -> dict:
+    """A transcript stream state (``stream_state()`` form) as fixture JSON."""
+    if kind == "blake3":
+        cv, stack, block, chunk, done = state
+        return {"kind": "blake3", "cv": list(cv), "stack": [list(c) for c in stack],
+                "block": bytes(block).hex(), "chunk_counter": chunk, "blocks_done": done}
+    h, pend, tot = state
+    return {"h": list(h), "pending": bytes(pend).hex(), "total": tot}
+
+
+def fixture_state(fx: dict) -> tuple:
+    """The expected ``stream_state()`` of a fixture's final transcript."""
+    t = fx["transcript"]
+    if t.get("kind") == "blake3":
+        return (tuple(t["cv"]), tuple(tuple(c) for c in t["stack"]), bytes.fromhex(t["block"]),
+                t["chunk_counter"], t["blocks_done"])
+    return (tuple(t["h"]), bytes.fromhex(t["pending"]), t["total"])

@@ -56,6 +56,7 @@ class OracleResult:
     h: tuple
     pending: bytes
     total: int
+    state: tuple = ()    # the transcript's stream_state() form (either hash)
 
 
 def sha_from_stream(stream: bytes) -> OrefSha:
@@ -76,7 +77,9 @@ def initial_stream(label: bytes) -> bytes:
 
 
 def run_c(inst, threads: int = 0, stream: bytes | None = None) -> OracleResult:
-    """Run the C oracle on a workloads.Instance."""
+    """Run the C oracle on a workloads.Instance (SHA-256 transcript)."""
+    if inst.transcript_kind != "sha256":
+        raise ValueError("the C oracle restates the SHA-256 transcript only")
     lib = oref()
     tf, cf, trf = inst.tables.field, inst.chal, inst.transcript_field
     eq = 1 if inst.mode == "zerocheck" else 0
@@ -97,8 +100,8 @@ def run_c(inst, threads: int = 0, stream: bytes | None = None) -> OracleResult:
                         ctypes.addressof(st), msgs.ctypes.data, point.ctypes.data,
                         value.ctypes.data, final.ctypes.data, threads)
     assert rc == 0, rc
-    return OracleResult(msgs, point, value, final[:k + eq], tuple(st.h),
-                        bytes(st.pending[:st.pending_len]), int(st.total_len))
+    h, pend, tot = tuple(st.h), bytes(st.pending[:st.pending_len]), int(st.total_len)
+    return OracleResult(msgs, point, value, final[:k + eq], h, pend, tot, (h, pend, tot))
 
 
 def run_py(inst) -> tuple[OracleResult, OS.OracleProof]:
@@ -108,7 +111,8 @@ def run_py(inst) -> tuple[OracleResult, OS.OracleProof]:
     tabs = [[tf.from_words([int(w) for w in inst.tables.words[q, x]])
              for x in range(inst.tables.shape[1])] for q in range(inst.tables.shape[0])]
     claim = cf.from_words([int(w) for w in inst.claim.words.reshape(-1)])
-    tr = OT.OracleTranscript.new(inst.label, trf.item_bytes)
+    tcls = OT.OracleBlake3Transcript if inst.transcript_kind == "blake3" else OT.OracleTranscript
+    tr = tcls.new(inst.label, trf.item_bytes)
     if inst.mode == "zerocheck":
         z = [cf.from_words([int(w) for w in inst.z.words[i]]) for i in range(inst.n)]
         pf = OS.prove_eq(cf, trf, tabs, z, claim, tr, inst.degree)
@@ -121,5 +125,6 @@ def run_py(inst) -> tuple[OracleResult, OS.OracleProof]:
     fin = [cf.to_words(t[0]) for t in pf.final_tables]
     if inst.mode == "zerocheck":
         fin.append(cf.to_words(pf.eq_prev))
-    h, pend, tot = pf.transcript.state()
-    return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot), pf
+    state = pf.transcript.state()
+    h, pend, tot = state if inst.transcript_kind == "sha256" else ((), b"", 0)
+    return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot, state), pf

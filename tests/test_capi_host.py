@@ -4,6 +4,7 @@ agree with the oracle."""
 
 from __future__ import annotations
 
+import hashlib
 import json
 import re
 from pathlib import Path
@@ -11,9 +12,11 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from oracle import blake3 as OB3
 from oracle import transcript as OT
-from tests.golden.recipes import RECIPES, build
+from tests.golden.recipes import RECIPES, build, fixture_state
 from zorch_amd import _lib
+from zorch_amd import workloads as W
 from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import (
     FieldArray,
@@ -23,6 +26,7 @@ from zorch_amd.fields import (
     from_storage,
     goldilocks,
 )
+from zorch_amd.blake3_field_transcript import Blake3FieldTranscript
 from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
 from zorch_amd.sumcheck.stage import SumClaim, SumcheckVerifier
 from zorch_amd.sumcheck.eq.stage import EqPolyVerifier, EqSumClaim
@@ -48,11 +52,14 @@ def _oracle_item(dtype):
     return 16 if dtype is babybearx4_mont else (8 if dtype is goldilocks else 4)
 
 
+@pytest.mark.parametrize("kind", ["sha256", "blake3"])
 @pytest.mark.parametrize("dtype", [babybear_mont, babybearx4_mont, goldilocks])
-def test_host_transcript_matches_oracle(dtype):
+def test_host_transcript_matches_oracle(dtype, kind):
     item = _oracle_item(dtype)
-    t = Sha256FieldTranscript.new(b"domain-label", dtype)
-    o = OT.OracleTranscript.new(b"domain-label", item)
+    cls, ocls = ((Blake3FieldTranscript, OT.OracleBlake3Transcript) if kind == "blake3"
+                 else (Sha256FieldTranscript, OT.OracleTranscript))
+    t = cls.new(b"domain-label", dtype)
+    o = ocls.new(b"domain-label", item)
     assert t.stream_state() == o.state()
     vals = array(list(range(1, 7)), dtype)
     t = t.observe(vals)
@@ -71,6 +78,17 @@ def test_host_transcript_matches_oracle(dtype):
     o, so2 = o.sample(0)
     assert s2.shape == (0,) and so2 == b""
     assert t.stream_state() == o.state()
+    # Long streams: chunk boundaries and subtree merges (BLAKE3), many blocks.
+    rng = np.random.default_rng(len(kind) + item)
+    for _ in range(40):
+        blob = rng.bytes(int(rng.integers(0, 400)))
+        t, o = t.observe_bytes(blob), o.observe_bytes(blob)
+        t, a = t.sample(int(rng.integers(1, 9)))
+        o, b = o.sample(a.shape[0])
+        assert a.tobytes() == b
+    assert t.stream_state() == o.state()
+    assert t.digest() == (OB3.blake3(o.stream) if kind == "blake3"
+                          else hashlib.sha256(o.stream).digest())
 
 
 def test_transcripts_are_immutable_values():
@@ -109,7 +127,7 @@ def test_host_verifier_accepts_oracle_proofs(name):
     fx, inst = _fixture(name)
     c = inst.chal
     proof = from_storage(np.array(fx["msgs"], dtype=c.word_dtype), c)
-    t0 = Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+    t0 = W.new_transcript(inst)
     pol = ChallengePolicy(c)
     if inst.mode == "zerocheck":
         v = EqPolyVerifier(ProductSummand(inst.degree), challenges=pol)
@@ -120,9 +138,7 @@ def test_host_verifier_accepts_oracle_proofs(name):
     assert res.ok
     assert res.reduced_claim.point.words.reshape(inst.n, -1).tolist() == fx["point"]
     assert res.reduced_claim.value.words.reshape(-1).tolist() == fx["value"]
-    h, pend, tot = res.transcript.stream_state()
-    assert list(h) == fx["transcript"]["h"] and pend.hex() == fx["transcript"]["pending"]
-    assert tot == fx["transcript"]["total"]
+    assert res.transcript.stream_state() == fixture_state(fx)
 
 
 def test_host_verifier_rejects_tampering_and_malformed():

@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.golden.recipes import RECIPES, build
+from tests.golden.recipes import RECIPES, build, fixture_state
 from tests.oracle_glue import run_c
 from zorch_amd import workloads as W
 from zorch_amd.challenge import ChallengePolicy
@@ -33,7 +33,7 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 def _prove(inst, tables=None):
-    t0 = Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+    t0 = W.new_transcript(inst)
     pol = ChallengePolicy(inst.chal)
     tables = inst.tables if tables is None else tables
     if inst.mode == "zerocheck":
@@ -44,7 +44,7 @@ def _prove(inst, tables=None):
 
 
 def _verify(inst, res):
-    t0 = Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+    t0 = W.new_transcript(inst)
     pol = ChallengePolicy(inst.chal)
     if inst.mode == "zerocheck":
         v = EqPolyVerifier(ProductSummand(inst.degree), challenges=pol)
@@ -57,13 +57,11 @@ def _words(fa: FieldArray):
     return fa.words.reshape(fa.words.shape[0], -1).tolist() if fa.ndim else fa.words.tolist()
 
 
-def _assert_matches(res, ref_msgs, ref_point, ref_value, ref_h, ref_pending, ref_total):
+def _assert_matches(res, ref_msgs, ref_point, ref_value, ref_state):
     assert res.reduction_proof.words.tolist() == ref_msgs
     assert res.reduced_claim.point.words.tolist() == ref_point
     assert res.reduced_claim.value.words.reshape(-1).tolist() == ref_value
-    h, pend, tot = res.transcript.stream_state()
-    assert list(h) == list(ref_h)
-    assert pend == ref_pending and tot == ref_total
+    assert res.transcript.stream_state() == tuple(ref_state)
 
 
 @pytest.mark.parametrize("name", sorted(RECIPES))
@@ -71,19 +69,19 @@ def test_stage_matches_golden_fixture(name):
     fx = json.loads((GOLDEN / f"{name}.json").read_text())
     inst = build(fx["recipe"])
     res = _prove(inst)
-    _assert_matches(res, fx["msgs"], fx["point"], fx["value"], fx["transcript"]["h"],
-                    bytes.fromhex(fx["transcript"]["pending"]), fx["transcript"]["total"])
+    _assert_matches(res, fx["msgs"], fx["point"], fx["value"], fixture_state(fx))
     assert _verify(inst, res).ok
 
 
 @pytest.mark.parametrize("name", ["product_bb_n4", "product_bb_k3_n4", "product_bb_bb4_n6",
                                   "zerocheck_bb4_n5", "zerocheck_bb4_t4_n5", "product_gl_n5",
-                                  "product_bb_n1", "zerocheck_bb4_n2"])
+                                  "product_bb_n1", "zerocheck_bb4_n2", "zerocheck_bb4_n9_b3",
+                                  "product_gl_n6_b3", "zerocheck_bb4_t4_n5_b3"])
 def test_round_by_round_matches_golden_fixture(name):
     """The drop-in per-round path under fold_rounds equals the whole stage."""
     fx = json.loads((GOLDEN / f"{name}.json").read_text())
     inst = build(fx["recipe"])
-    t0 = Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+    t0 = W.new_transcript(inst)
     pol = ChallengePolicy(inst.chal)
     if inst.mode == "zerocheck":
         rnd = EqPolyRound(ProductSummand(inst.degree), inst.z, challenges=pol)
@@ -96,8 +94,7 @@ def test_round_by_round_matches_golden_fixture(name):
     assert carry.claim.index == inst.n
     final = carry.state.read_tables()
     assert final.words[:, 0].tolist() == fx["final"][: inst.degree]
-    h, pend, tot = t.stream_state()
-    assert list(h) == fx["transcript"]["h"] and tot == fx["transcript"]["total"]
+    assert t.stream_state() == fixture_state(fx)
     # A consumed carry cannot be replayed.
     with pytest.raises(ValueError):
         rnd(initial_claim(carry.state, inst.claim, inst.n), t0)
@@ -106,8 +103,7 @@ def test_round_by_round_matches_golden_fixture(name):
 def _compare_with_c(inst, threads=16):
     res = _prove(inst)
     ref = run_c(inst, threads=threads)
-    _assert_matches(res, ref.msgs.tolist(), ref.point.tolist(), ref.value.tolist(), ref.h,
-                    ref.pending, ref.total)
+    _assert_matches(res, ref.msgs.tolist(), ref.point.tolist(), ref.value.tolist(), ref.state)
     return res
 
 

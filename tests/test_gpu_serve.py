@@ -16,7 +16,6 @@ from zorch_amd import _lib
 from zorch_amd import workloads as W
 from zorch_amd.runtime import Context, DeviceTables
 from zorch_amd.serve import ProofServer
-from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
 from zorch_amd.sumcheck._device import prove_stage
 
 pytestmark = pytest.mark.gpu
@@ -27,7 +26,7 @@ def _mode(inst):
 
 
 def _t0(inst):
-    return Sha256FieldTranscript.new(inst.label, inst.transcript_field)
+    return W.new_transcript(inst)
 
 
 def _submit(srv, inst, tables=None):
@@ -55,6 +54,9 @@ def _requests():
     insts.append(W.product_bb_ext(13, (1401, 1402), 1403))
     insts.append(W.product_bb(12, (1404, 1405, 1406), 1407))
     insts.append(W.product_gl(14, (1411, 1412), 1413))
+    b3 = W.zerocheck_bb4(12, (1421, 1422), 1423, 1424)
+    b3.transcript_kind = "blake3"
+    insts.append(b3)
     return insts
 
 

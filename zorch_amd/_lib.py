@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libzorch_amd.so"
 
 ZS_OK, ZS_EINVAL, ZS_ENOMEM, ZS_EHIP, ZS_EBUDGET, ZS_ENODEV = 0, 1, 2, 3, 4, 5
 ZS_F_BABYBEAR, ZS_F_BABYBEAR4, ZS_F_GOLDILOCKS = 1, 2, 3
-ZS_TR_SHA256 = 1
+ZS_TR_SHA256, ZS_TR_BLAKE3 = 1, 2
 ZS_MODE_PRODUCT, ZS_MODE_EQ = 1, 2
 
 
@@ -28,13 +28,31 @@ class Sha256StateC(ctypes.Structure):
     ]
 
 
+class Blake3StateC(ctypes.Structure):
+    _fields_ = [
+        ("cv", ctypes.c_uint32 * 8),
+        ("stack", (ctypes.c_uint32 * 8) * 16),
+        ("block", ctypes.c_uint8 * 64),
+        ("chunk_counter", ctypes.c_uint64),
+        ("block_len", ctypes.c_uint32),
+        ("blocks_done", ctypes.c_uint32),
+        ("stack_len", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class _StreamC(ctypes.Union):
+    _fields_ = [("sha", Sha256StateC), ("blake3", Blake3StateC)]
+
+
 class TranscriptC(ctypes.Structure):
+    _anonymous_ = ("stream",)
     _fields_ = [
         ("kind", ctypes.c_uint32),
         ("field", ctypes.c_uint32),
         ("item_bytes", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
-        ("sha", Sha256StateC),
+        ("stream", _StreamC),
     ]
 
     def copy(self) -> "TranscriptC":
@@ -96,7 +114,8 @@ class ServerStatsC(ctypes.Structure):
 
 
 assert ctypes.sizeof(Sha256StateC) == 112
-assert ctypes.sizeof(TranscriptC) == 128
+assert ctypes.sizeof(Blake3StateC) == 632
+assert ctypes.sizeof(TranscriptC) == 648
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32

@@ -228,7 +228,7 @@ class BatchGroupImpl final : public BatchGroup {
       if (o.value) memcpy(o.value, &c.claim, sizeof(TC));
       if (o.final_tables)
         memcpy(o.final_tables, rec + o_final(m.n), (K + (EQ ? 1 : 0)) * sizeof(TC));
-      if (ts) memcpy(&ts[m.index].sha, &c.sha, sizeof(Sha256State));
+      if (ts) memcpy(&ts[m.index].sha, &c.tr, sizeof(TrState));
     }
     return ZS_OK;
   }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/zorch_amd.h"
+#include "blake3.h"
 #include "field.h"
 #include "runtime.h"
 #include "sha256.h"
@@ -115,6 +116,18 @@ uint32_t field_bytes(uint32_t field) {
   }
 }
 
+static bool known_kind(uint32_t kind) { return kind == ZS_TR_SHA256 || kind == ZS_TR_BLAKE3; }
+
+// One Fiat-Shamir hop (observe under slice framing, sample `limbs` words) on
+// either transcript kind; false if a BLAKE3 stream outgrows its subtree stack.
+static bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, size_t item,
+                           size_t limbs, uint8_t* out) {
+  if (t->kind == ZS_TR_BLAKE3)
+    return b3_fs_hop(*reinterpret_cast<Blake3State*>(&t->blake3), msg, len, item, limbs, out);
+  sha256_fs_hop(*reinterpret_cast<Sha256State*>(&t->sha), msg, len, item, limbs, out);
+  return true;
+}
+
 // Base prime field of a field id.
 static uint32_t base_of(uint32_t f) { return f == ZS_F_BABYBEAR4 ? ZS_F_BABYBEAR : f; }
 
@@ -134,7 +147,7 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
   if (!a.tables || !a.claim) return fail(ZS_EINVAL, "null tables or claim");
   if (a.mode == ZS_MODE_EQ && !a.eq_point) return fail(ZS_EINVAL, "eq mode needs an equality point");
   if (t) {
-    if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+    if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
     const uint32_t tf = t->field;
     if (!(tf == a.chal_field || tf == base_of(a.chal_field)))
       return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
@@ -188,9 +201,9 @@ static int verify_round_typed(zs_transcript* t, const void* claim_p, const void*
   const size_t item = t->item_bytes;
   const size_t limbs = sizeof(T) / item;
   std::vector<uint8_t> raw(sizeof(T));
-  Sha256State* s = reinterpret_cast<Sha256State*>(&t->sha);
-  sha256_fs_hop(*s, reinterpret_cast<const uint8_t*>(msg_p), npts * sizeof(T), item, limbs,
-                raw.data());
+  if (!transcript_hop(t, reinterpret_cast<const uint8_t*>(msg_p), npts * sizeof(T), item, limbs,
+                      raw.data()))
+    return fail(ZS_EINVAL, "transcript stream exceeds the BLAKE3 subtree stack");
   if (chal_out) memcpy(chal_out, raw.data(), sizeof(T));
   T r;
   memcpy(&r, raw.data(), sizeof(T));
@@ -418,34 +431,62 @@ int zs_ctx_profile_read(zs_ctx* ctx, zs_kernel_record* out, uint32_t cap, uint32
 int zs_transcript_new(uint32_t kind, uint32_t field, const uint8_t* domain, size_t len,
                       zs_transcript* out) {
   if (!out || (len && !domain)) return fail(ZS_EINVAL, "null argument");
-  if (kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (!known_kind(kind)) return fail(ZS_EINVAL, "unknown transcript kind");
   if (!field_bytes(field)) return fail(ZS_EINVAL, "unknown field id");
   memset(out, 0, sizeof(*out));
   out->kind = kind;
   out->field = field;
   out->item_bytes = field_bytes(field);
+  uint8_t head[9];
+  head[0] = OP_DOMAIN;
+  for (int i = 0; i < 8; ++i) head[1 + i] = (uint8_t)((uint64_t)len >> (8 * i));
+  if (kind == ZS_TR_BLAKE3) {
+    Blake3State* s = reinterpret_cast<Blake3State*>(&out->blake3);
+    b3_init(*s);
+    if (!b3_absorb(*s, head, 9) || !b3_absorb(*s, domain, len))
+      return fail(ZS_EINVAL, "transcript stream exceeds the BLAKE3 subtree stack");
+    return ZS_OK;
+  }
   Sha256State* s = reinterpret_cast<Sha256State*>(&out->sha);
   sha256_init(*s);
-  sha256_absorb_byte(*s, OP_DOMAIN);
-  sha256_absorb_le64(*s, len);
+  sha256_absorb(*s, head, 9);
   sha256_absorb(*s, domain, len);
   return ZS_OK;
 }
 
 int zs_transcript_absorb(zs_transcript* t, const uint8_t* data, size_t len) {
   if (!t || (len && !data)) return fail(ZS_EINVAL, "null argument");
+  if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (t->kind == ZS_TR_BLAKE3) {
+    if (!b3_absorb(*reinterpret_cast<Blake3State*>(&t->blake3), data, len))
+      return fail(ZS_EINVAL, "transcript stream exceeds the BLAKE3 subtree stack");
+    return ZS_OK;
+  }
   sha256_absorb(*reinterpret_cast<Sha256State*>(&t->sha), data, len);
   return ZS_OK;
 }
 
 int zs_transcript_squeeze(zs_transcript* t, size_t nbytes, uint8_t* out) {
   if (!t || (nbytes && !out)) return fail(ZS_EINVAL, "null argument");
+  if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (t->kind == ZS_TR_BLAKE3) {
+    Blake3State* s = reinterpret_cast<Blake3State*>(&t->blake3);
+    b3_finalize(*s, out, nbytes);
+    if (!b3_absorb(*s, out, nbytes))
+      return fail(ZS_EINVAL, "transcript stream exceeds the BLAKE3 subtree stack");
+    return ZS_OK;
+  }
   sha256_squeeze(*reinterpret_cast<Sha256State*>(&t->sha), out, nbytes);
   return ZS_OK;
 }
 
 int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]) {
   if (!t || !out) return fail(ZS_EINVAL, "null argument");
+  if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (t->kind == ZS_TR_BLAKE3) {
+    b3_finalize(*reinterpret_cast<const Blake3State*>(&t->blake3), out, 32);
+    return ZS_OK;
+  }
   sha256_digest_suffix(*reinterpret_cast<const Sha256State*>(&t->sha), nullptr, 0, out);
   return ZS_OK;
 }
@@ -530,7 +571,7 @@ int zs_state_round(zs_state* st, zs_transcript* t, void* msg_out, void* chal_out
                    void* claim_out) {
   if (!st || !t) return fail(ZS_EINVAL, "null argument");
   if (st->bound >= st->inst->rounds) return fail(ZS_EINVAL, "every round is already bound");
-  if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
   if (t->item_bytes == 0 || st->inst->chal_bytes % t->item_bytes)
     return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
   int rc;
@@ -572,7 +613,7 @@ int zs_verify_round(uint32_t chal_field, zs_transcript* t, const void* claim, co
   if (npts != degree + 1)
     return fail(ZS_EINVAL, "round message must have " + std::to_string(degree + 1) +
                                " evals, got " + std::to_string(npts));
-  if (t->kind != ZS_TR_SHA256) return fail(ZS_EINVAL, "unknown transcript kind");
+  if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
   const uint32_t cb = field_bytes(chal_field);
   if (!cb) return fail(ZS_EINVAL, "unknown field id");
   if (!(t->field == chal_field || t->field == base_of(chal_field)) || t->item_bytes == 0 ||

@@ -1,17 +1,21 @@
 // Fiat-Shamir hop run by one full wavefront (64 lanes) on the device.
 //
 // The SHA-256 chain of a transcript is strictly sequential, so one wave runs
-// it; the lanes share the byte work around the compressions: absorbing copies
-// up to 64 bytes into the pending block in one step (one byte per lane), the
-// padded tail block(s) of a squeeze are assembled one byte per lane, and the
-// compression reads its 16 message words from LDS as aligned dwords. The
-// chaining value, pending length and total length are held uniformly in every
-// lane's registers; only the 64-byte pending block lives in LDS.
-//
-// A hop is a small schedule of compressions (absorb the framed message, one
-// or two tail blocks per squeeze counter, absorb the squeezed bytes). They all
-// go through one call site of the fully unrolled compression, so the 64-round
-// body is fetched into the instruction cache once per hop and then reused.
+// it. A single wave issues one instruction every four cycles, so the hop's
+// time is its instruction count; it is cut two ways:
+//  * Every block whose bytes are known when the hop starts (the blocks that
+//    absorb the framed message, and the padded tail block(s) of the squeeze,
+//    which hold the new pending bytes, the counter and the length) has its
+//    message schedule expanded first, lane b expanding block b, into
+//    W_t + K_t words in LDS. The chain of compressions then runs the 64
+//    rounds alone, reading four words per LDS load. Only the block that
+//    absorbs the squeezed bytes (when the pending bytes overflow) depends on
+//    the digest; its schedule is expanded when the digest is known.
+//  * The chaining values stay in vector registers (vreg): left to itself the
+//    compiler runs wave-uniform arithmetic on the scalar unit, which lacks
+//    rotates and three-input operations.
+// All compressions go through one call site, so the unrolled 64-round body is
+// fetched into the instruction cache once per hop.
 //
 // Byte stream identical to sha256_fs_hop (sha256.h): observe(msg) under slice
 // framing, sample(limbs) (reference zorch/sha256_field_transcript.py:100-114,
@@ -46,37 +50,70 @@ __device__ __forceinline__ uint32_t vreg(uint32_t x) {
   return y;
 }
 
-// Fill the padded tail of SHA256(stream || le64(ctr)) into `tail` (128 B of
-// LDS), one byte per lane; returns the number of tail blocks (1 or 2).
-__device__ __forceinline__ uint32_t ws_build_tail(const WaveSha& s, const uint8_t* pend,
-                                                  uint64_t ctr, uint8_t* tail) {
+// Blocks of one hop: at most two absorbing the framed message (pending < 64
+// bytes plus at most 20 + 96 framed bytes), two squeeze tail blocks and one
+// absorbing the squeezed bytes.
+constexpr uint32_t HOP_MAX_BLOCKS = 5;
+constexpr uint32_t HOP_MAX_MSG = 96;
+
+// Lanes b < nblk expand block first + b (64 bytes at blocks + 64 (first + b))
+// into wk[64 (first + b) + t] = W_t + K_t, t = 0..63.
+__device__ __forceinline__ void ws_schedule(const uint8_t* blocks, uint32_t first, uint32_t nblk,
+                                            uint32_t* wk) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t plen = s.plen, mlen = plen + 8u;
-  const uint32_t nblk = (mlen + 9u <= 64u) ? 1u : 2u;
-  const uint64_t bits = (s.total + 8u) * 8u;
-  for (uint32_t half = 0; half < nblk; ++half) {
-    const uint32_t pos = half * 64u + lane;
-    uint32_t b;
-    if (pos < plen) b = pend[pos];
-    else if (pos < mlen) b = (uint32_t)(ctr >> (8u * (pos - plen))) & 0xffu;
-    else if (pos == mlen) b = 0x80u;
-    else if (pos >= nblk * 64u - 8u) b = (uint32_t)(bits >> (8u * (nblk * 64u - 1u - pos))) & 0xffu;
-    else b = 0;
-    tail[pos] = (uint8_t)b;
+  if (lane < nblk) {
+    const uint32_t b = first + lane;
+    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(blocks + 64u * b);
+    uint32_t* o = wk + 64u * b;
+    uint32_t w[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) w[t] = __builtin_bswap32(b32[t]);  // all loads before any store
+#pragma unroll
+    for (int t = 0; t < 16; ++t) o[t] = w[t] + ZS_K256[t];
+#pragma unroll
+    for (int t = 16; t < 64; ++t) {
+      const uint32_t x = w[(t - 15) & 15], y = w[(t - 2) & 15];
+      const uint32_t s0 = sha_xor3(sha_rotr(x, 7), sha_rotr(x, 18), x >> 3);
+      const uint32_t s1 = sha_xor3(sha_rotr(y, 17), sha_rotr(y, 19), y >> 10);
+      const uint32_t wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
+      w[t & 15] = wt;
+      o[t] = wt + ZS_K256[t];
+    }
   }
   wave_sync();
-  return nblk;
 }
 
-// observe(msg) + sample(limbs): `msg` (LDS, msg_len bytes) is framed into
-// `buf` (LDS, >= msg_len + 20 bytes); the squeezed bytes land in `out` (LDS,
-// 32-byte granules), `tail` (LDS, 128 B) holds squeeze tail blocks and `dgw`
-// (LDS, 8 words) the chaining value between the two blocks of a squeeze
-// tail, so that only the stream state occupies registers across compressions.
+// The 64 rounds of one compression from an expanded schedule (LDS).
+__device__ __forceinline__ void sha256_rounds(uint32_t h[8], const uint32_t* wk) {
+  uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+  const uint4* q = reinterpret_cast<const uint4*>(wk);
+#pragma unroll
+  for (int t4 = 0; t4 < 16; ++t4) {
+    const uint4 v = q[t4];
+    const uint32_t kw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t S1 = sha_xor3(sha_rotr(e, 6), sha_rotr(e, 11), sha_rotr(e, 25));
+      const uint32_t ch = __builtin_amdgcn_bitop3_b32(e, f, g, 0xCA);  // e ? f : g
+      const uint32_t t1 = hh + S1 + ch + kw[u];
+      const uint32_t S0 = sha_xor3(sha_rotr(a, 2), sha_rotr(a, 13), sha_rotr(a, 22));
+      const uint32_t mj = __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);  // majority
+      hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + S0 + mj;
+    }
+  }
+  h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
+// observe(msg) + sample(limbs), with limbs * item <= 32 (one squeeze counter).
+// `msg` (LDS, msg_len <= HOP_MAX_MSG bytes) is framed into `buf` (LDS,
+// >= msg_len + 20 bytes); `blocks` (LDS, HOP_MAX_BLOCKS x 64 B) and `wk`
+// (LDS, HOP_MAX_BLOCKS x 64 words, 16-byte aligned) hold the hop's blocks and
+// schedules, `dgw` (LDS, 8 words) the digest; the squeezed bytes land in
+// `out` (LDS). The stream's pending bytes live in `pend` (LDS, 64 B).
 __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* buf,
                                           const uint8_t* msg, uint32_t msg_len, uint32_t item,
-                                          uint32_t limbs, uint8_t* tail, uint32_t* dgw,
-                                          uint8_t* out) {
+                                          uint32_t limbs, uint8_t* blocks, uint32_t* wk,
+                                          uint32_t* dgw, uint8_t* out) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t count = msg_len / item;
   const uint32_t framed = msg_len + 20u;
@@ -92,76 +129,86 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
     buf[pos] = (uint8_t)b;
   }
   wave_sync();
+  // Absorb stream = pending || framed: `na` full blocks, then `rem` bytes
+  // pending; the squeeze tail block(s) start with those rem bytes.
   const uint32_t nbytes = item * limbs;
-  // phase 0: absorb buf; 1: squeeze tail blocks; 2: absorb out; 3: done.
-  uint32_t phase = 0;
-  const uint8_t* src = buf;
-  uint32_t src_len = framed, src_pos = 0;
-  uint32_t ctr = 0, dblk = 0, dnblk = 0, got = 0;
-  while (phase != 3u) {
-    const uint8_t* blk;
-    bool squeeze;
-    if (phase != 1u) {
-      const uint32_t room = 64u - s.plen;
-      const uint32_t take = (src_len - src_pos) < room ? (src_len - src_pos) : room;
-      if (lane < take) pend[s.plen + lane] = src[src_pos + lane];
-      wave_sync();
-      s.plen += take;
-      s.total += take;
-      src_pos += take;
-      if (s.plen < 64u) {  // source exhausted, block not full
-        if (phase == 0u) {
-          phase = 1u;
-          dblk = 0;
-          continue;
-        }
-        phase = 3u;
-        break;
-      }
-      blk = pend;
-      squeeze = false;
+  const uint32_t plen = s.plen;
+  const uint32_t total = plen + framed;
+  const uint32_t na = total / 64u, rem = total % 64u;
+  const uint64_t absorbed = s.total + framed;
+  const uint32_t ntail = (rem + 8u + 9u <= 64u) ? 1u : 2u;
+  const uint32_t nfinal = (rem + nbytes >= 64u) ? 1u : 0u;
+  const uint64_t bits = (absorbed + 8u) * 8u;  // stream || le64(ctr = 0)
+  const uint32_t tail_end = 64u * (na + ntail);
+  for (uint32_t pos = lane; pos < tail_end; pos += 64u) {
+    uint32_t b;
+    if (pos < total) {
+      b = pos < plen ? pend[pos] : buf[pos - plen];
     } else {
-      if (dblk == 0) dnblk = ws_build_tail(s, pend, ctr, tail);
-      blk = tail + 64u * dblk;
-      squeeze = true;
+      const uint32_t p = pos - 64u * na;  // position in the tail
+      if (p < rem + 8u) b = 0;            // le64(0)
+      else if (p == rem + 8u) b = 0x80u;
+      else if (p >= 64u * ntail - 8u) b = (uint32_t)(bits >> (8u * (64u * ntail - 1u - p))) & 0xffu;
+      else b = 0;
+    }
+    blocks[pos] = (uint8_t)b;
+  }
+  wave_sync();
+  ws_schedule(blocks, 0, na + ntail, wk);
+  const uint32_t njobs = na + ntail + nfinal;
+  uint32_t h[8], d[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    h[i] = s.h[i];
+    d[i] = 0;
+  }
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const bool tail = j >= na && j < na + ntail;
+    if (j == na + ntail) {
+      // Absorb the squeezed bytes: pending (rem bytes, the head of the first
+      // tail block) || out[0 .. 64 - rem) fills one more block.
+      blocks[64u * j + lane] = lane < rem ? blocks[64u * na + lane] : out[lane - rem];
+      wave_sync();
+      ws_schedule(blocks, j, 1, wk);
     }
     // The one compression site.
-    uint32_t w[16], hh[8];
-    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(blk);
+    uint32_t hh[8];
+    const bool from_d = tail && j > na;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = vreg(__builtin_bswap32(b32[i]));
-    const bool from_dg = squeeze && dblk > 0;
+    for (int i = 0; i < 8; ++i) hh[i] = vreg(from_d ? d[i] : h[i]);
+    sha256_rounds(hh, wk + 64u * j);
+    if (tail) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) hh[i] = vreg(from_dg ? dgw[i] : s.h[i]);
-    sha256_compress_words(hh, w);
-    wave_sync();
-    if (!squeeze) {
+      for (int i = 0; i < 8; ++i) d[i] = hh[i];
+    } else {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) s.h[i] = hh[i];
-      s.plen = 0;
-      continue;
+      for (int i = 0; i < 8; ++i) h[i] = hh[i];
     }
-    if (lane == 0) {
+    if (j == na + ntail - 1u) {
+      // Digest complete: its first nbytes (big-endian words) are the output.
+      if (lane == 0) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) dgw[i] = hh[i];
-    }
-    wave_sync();
-    if (++dblk < dnblk) continue;
-    // Digest complete: its first bytes (big-endian words) are squeeze output.
-    const uint32_t take = (nbytes - got) < 32u ? (nbytes - got) : 32u;
-    if (lane < take)
-      out[got + lane] = (uint8_t)(dgw[lane >> 2] >> (24u - 8u * (lane & 3u)));
-    wave_sync();
-    got += take;
-    ++ctr;
-    dblk = 0;
-    if (got >= nbytes) {
-      phase = 2u;
-      src = out;
-      src_len = nbytes;
-      src_pos = 0;
+        for (int i = 0; i < 8; ++i) dgw[i] = d[i];
+      }
+      wave_sync();
+      if (lane < nbytes) out[lane] = (uint8_t)(dgw[lane >> 2] >> (24u - 8u * (lane & 3u)));
+      wave_sync();
     }
   }
+  // New pending bytes: rem || out, minus the block absorbed when it filled.
+  uint8_t pb = 0;
+  const uint32_t plen2 = rem + nbytes - 64u * nfinal;
+  if (lane < plen2) {
+    const uint32_t p = lane + 64u * nfinal;  // position in rem || out
+    pb = p < rem ? blocks[64u * na + p] : out[p - rem];
+  }
+  wave_sync();
+  pend[lane] = pb;
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s.h[i] = h[i];
+  s.plen = plen2;
+  s.total = absorbed + nbytes;
 }
 
 }  // namespace zs

@@ -52,7 +52,10 @@ void init_ctl_consts(Ctl<FC>& c) {
 // The per-proof part of a control block: transcript, claim, running state.
 template <class FC>
 void set_ctl_proof(Ctl<FC>& c, const zs_transcript& t, const void* claim) {
-  memcpy(&c.sha, &t.sha, sizeof(Sha256State));
+  static_assert(sizeof(TrState) == sizeof(zs_transcript) - offsetof(zs_transcript, sha),
+                "transcript state layout");
+  memcpy(&c.tr, &t.sha, sizeof(TrState));
+  c.tr_kind = t.kind;
   c.item_bytes = t.item_bytes;
   c.limbs = (uint32_t)(sizeof(typename FC::T) / t.item_bytes);
   c.counter = 0;
@@ -233,14 +236,15 @@ class InstanceImpl final : public Instance {
     if (out->point) memcpy(out->point, h + o_point_, (size_t)n_ * sizeof(TC));
     if (out->value) memcpy(out->value, &c.claim, sizeof(TC));
     if (out->final_tables) memcpy(out->final_tables, h + o_final_, (K + (EQ ? 1 : 0)) * sizeof(TC));
-    if (t) memcpy(&t->sha, &c.sha, sizeof(Sha256State));
+    if (t) memcpy(&t->sha, &c.tr, sizeof(TrState));
     return ZS_OK;
   }
 
   int upload_transcript(const zs_transcript& t, hipStream_t s) override {
     // The transcript prefix of the control block: state, word size, limbs.
     Ctl<FC>& c = *(Ctl<FC>*)((uint8_t*)host_ + o_ctl_);
-    memcpy(&c.sha, &t.sha, sizeof(Sha256State));
+    memcpy(&c.tr, &t.sha, sizeof(TrState));
+    c.tr_kind = t.kind;
     c.item_bytes = t.item_bytes;
     c.limbs = (uint32_t)(sizeof(TC) / t.item_bytes);
     static_assert(offsetof(Ctl<FC>, limbs) + sizeof(uint32_t) == offsetof(Ctl<FC>, counter),
@@ -257,7 +261,7 @@ class InstanceImpl final : public Instance {
     if (msg) memcpy(msg, h + o_msgs_ + (size_t)round * NPTS * sizeof(TC), NPTS * sizeof(TC));
     if (chal) memcpy(chal, h + o_point_ + (size_t)round * sizeof(TC), sizeof(TC));
     if (claim) memcpy(claim, &c.claim, sizeof(TC));
-    if (t) memcpy(&t->sha, &c.sha, sizeof(Sha256State));
+    if (t) memcpy(&t->sha, &c.tr, sizeof(TrState));
     return ZS_OK;
   }
 

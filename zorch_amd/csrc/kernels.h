@@ -41,6 +41,7 @@
 
 #include <type_traits>
 
+#include "blake3.h"
 #include "field.h"
 #include "hop.h"
 #include "sha256.h"
@@ -55,14 +56,21 @@ constexpr int MAXG = 4096;    // max blocks of one pass
 constexpr int NSLOT = 32;     // slot rows of the cross-block node sums
 constexpr int SLOTW = 32;     // 64-bit words per slot row (>= (K+1) x words/element)
 
+// Transcript kinds (the C ABI's ZS_TR_*) and their streaming states.
+constexpr uint32_t TR_SHA256 = 1, TR_BLAKE3 = 2;
+union TrState {
+  Sha256State sha;
+  Blake3State b3;
+};
+
 // Device control block of one proof (lives in device memory).
 template <class FC>
 struct Ctl {
-  Sha256State sha;
+  TrState tr;            // transcript stream
+  uint32_t tr_kind;      // TR_SHA256 or TR_BLAKE3
   uint32_t item_bytes;   // transcript word size
   uint32_t limbs;        // transcript words per challenge
   uint32_t counter;      // last-block ticket of the current pass
-  uint32_t pad;
   typename FC::T claim;
   typename FC::T r;        // current challenge, canonical
   typename FC::T eq_prev;  // eq mass of the bound prefix (eq mode)
@@ -263,6 +271,33 @@ struct PassUnroll {
   static constexpr int value = bytes >= 64 ? 1 : bytes >= 32 ? 2 : 4;
 };
 
+// Sum of w * x over a thread's pairs (w a challenge-field weight, x a value
+// of the pass field). BabyBear^4 weights on BabyBear values keep four 64-bit
+// limb sums of raw products (each < p^2, folded below p 2^32 after every
+// addition, so nothing overflows) and reduce once at the end: one multiply-add
+// per limb instead of a Montgomery product and a modular addition.
+template <class FC, class FV>
+struct WeightedSum {
+  typename FC::T s;
+  __device__ __forceinline__ void init() { s = FC::zero(); }
+  __device__ __forceinline__ void add(typename FC::T w, typename FV::T x) {
+    s = FC::add(s, MixMul<FC, FV>::go(w, x));
+  }
+  __device__ __forceinline__ typename FC::T get() const { return s; }
+};
+template <>
+struct WeightedSum<BB4, BB> {
+  uint64_t s[4];
+  __device__ __forceinline__ void init() { s[0] = s[1] = s[2] = s[3] = 0; }
+  __device__ __forceinline__ void add(bb4 w, uint32_t x) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[c] = bb_pre(s[c] + (uint64_t)w.c[c] * x);
+  }
+  __device__ __forceinline__ bb4 get() const {
+    return bb4_make(bb_redc(s[0]), bb_redc(s[1]), bb_redc(s[2]), bb_redc(s[3]));
+  }
+};
+
 template <class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void pass_accumulate(
     const typename FS::T* src, size_t src_stride, typename FC::T* dst, size_t dst_stride,
@@ -274,10 +309,17 @@ __device__ __forceinline__ void pass_accumulate(
   using S = typename FS::T;
   constexpr bool WEIGHT_FIRST = EQ && std::is_same<FV, FC>::value;
   constexpr int NS = FOLD ? 4 : 2;  // source streams per table
-  constexpr int U = PassUnroll<FS, K, FOLD>::value;
+  // The 64-bit weighted sums of BabyBear^4 weights on BabyBear values take
+  // the registers of two more unrolled pairs.
+  constexpr bool WIDE_SUMS = EQ && !WEIGHT_FIRST && std::is_same<FC, BB4>::value;
+  constexpr int U0 = PassUnroll<FS, K, FOLD>::value;
+  constexpr int U = WIDE_SUMS && U0 > 2 ? 2 : U0;
   const size_t half = L >> 1;
   const size_t nxr_mask = ((size_t)1 << nxr_log) - 1;
   const bool factored = EQ && wl != nullptr && (stride & nxr_mask) == 0;
+  WeightedSum<FC, FV> wsum[K + 1];
+#pragma unroll
+  for (int t = 0; t <= K; ++t) wsum[t].init();
   for (size_t j0 = start; j0 < half; j0 += (size_t)U * stride) {
     S raw[U][K][NS];
 #pragma unroll
@@ -346,7 +388,7 @@ __device__ __forceinline__ void pass_accumulate(
             V prod = lo[0];
 #pragma unroll
             for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-            acc[t] = FC::add(acc[t], MixMul<FC, FV>::go(w, prod));
+            wsum[t].add(w, prod);
             if (t < K) {
 #pragma unroll
               for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
@@ -370,6 +412,10 @@ __device__ __forceinline__ void pass_accumulate(
       // would only raise register pressure (their loads are already issued).
       __builtin_amdgcn_sched_barrier(0);
     }
+  }
+  if constexpr (EQ && !WEIGHT_FIRST) {
+#pragma unroll
+    for (int t = 0; t <= K; ++t) acc[t] = FC::add(acc[t], wsum[t].get());
   }
   if (factored && start < half) {
     const T wrc = wr[start & nxr_mask];
@@ -422,7 +468,8 @@ __device__ __forceinline__ uint64_t vreg_el<GL>(uint64_t v) {
 template <class FC>
 struct FinishLds {
   alignas(16) uint8_t pend[64];
-  alignas(16) uint8_t tail[128];
+  alignas(16) uint8_t blocks[HOP_MAX_BLOCKS * 64];
+  alignas(16) uint32_t wk[HOP_MAX_BLOCKS * 64];
   alignas(16) typename FC::T msg[MAXPTS];
   alignas(16) uint8_t buf[MAXPTS * 16 + 32];
   alignas(16) uint8_t out[64];
@@ -430,6 +477,7 @@ struct FinishLds {
   uint32_t h[8];
   uint32_t plen;
   uint64_t total;
+  Blake3State b3;  // the whole stream state of a BLAKE3 transcript
   typename FC::T claim, r, eq_prev;
 };
 
@@ -437,11 +485,19 @@ struct FinishLds {
 template <class FC>
 __device__ __forceinline__ void fin_load(FinishLds<FC>& sm, const Ctl<FC>* ctl) {
   const uint32_t lane = threadIdx.x & 63;
-  sm.pend[lane] = ctl->sha.pending[lane];
-  if (lane < 8u) sm.h[lane] = ctl->sha.h[lane];
+  if (ctl->tr_kind == TR_BLAKE3) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&ctl->tr.b3);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&sm.b3);
+    for (uint32_t i = lane; i < sizeof(Blake3State) / 4; i += 64u) dst[i] = src[i];
+  } else {
+    sm.pend[lane] = ctl->tr.sha.pending[lane];
+    if (lane < 8u) sm.h[lane] = ctl->tr.sha.h[lane];
+  }
   if (lane == 0) {
-    sm.plen = (uint32_t)ctl->sha.pending_len;
-    sm.total = ctl->sha.total_len;
+    if (ctl->tr_kind != TR_BLAKE3) {
+      sm.plen = (uint32_t)ctl->tr.sha.pending_len;
+      sm.total = ctl->tr.sha.total_len;
+    }
     sm.claim = ctl->claim;
     sm.r = ctl->r;
     sm.eq_prev = ctl->eq_prev;
@@ -454,12 +510,23 @@ __device__ __forceinline__ void fin_load(FinishLds<FC>& sm, const Ctl<FC>* ctl) 
 template <class FC>
 __device__ __forceinline__ void fin_store(const FinishLds<FC>& sm, Ctl<FC>* ctl) {
   const uint32_t lane = threadIdx.x & 63;
+  if (ctl->tr_kind == TR_BLAKE3) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&sm.b3);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&ctl->tr.b3);
+    for (uint32_t i = lane; i < sizeof(Blake3State) / 4; i += 64u) dst[i] = src[i];
+    if (lane == 0) {
+      ctl->claim = sm.claim;
+      ctl->r = sm.r;
+      ctl->eq_prev = sm.eq_prev;
+    }
+    return;
+  }
   const uint32_t plen = sm.plen;
-  ctl->sha.pending[lane] = lane < plen ? sm.pend[lane] : 0;
-  if (lane < 8u) ctl->sha.h[lane] = sm.h[lane];
+  ctl->tr.sha.pending[lane] = lane < plen ? sm.pend[lane] : 0;
+  if (lane < 8u) ctl->tr.sha.h[lane] = sm.h[lane];
   if (lane == 0) {
-    ctl->sha.pending_len = plen;
-    ctl->sha.total_len = sm.total;
+    ctl->tr.sha.pending_len = plen;
+    ctl->tr.sha.total_len = sm.total;
     ctl->claim = sm.claim;
     ctl->r = sm.r;
     ctl->eq_prev = sm.eq_prev;
@@ -481,6 +548,7 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
   constexpr int EB = sizeof(T);
   static_assert(NPTS <= 8, "one node per lane of an eight-lane group");
+  static_assert(NPTS * EB <= (int)HOP_MAX_MSG, "hop message bound");
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t t = lane & 7;
   const uint32_t tn = t < (uint32_t)NPTS ? t : (uint32_t)(NPTS - 1);  // in-bounds node index
@@ -508,14 +576,20 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   }
   wave_sync();
   if (lane == 0) trace_mark(a.trace, round, 4);
-  {
+  if (ctl->tr_kind == TR_BLAKE3) {
+    // Every lane runs the same byte-serial stream code on the LDS state.
+    b3_fs_hop(sm.b3, reinterpret_cast<const uint8_t*>(sm.msg), (size_t)(NPTS * EB),
+              ctl->item_bytes, ctl->limbs, sm.out);
+    wave_sync();
+  } else {
     WaveSha ws;
 #pragma unroll
     for (int i = 0; i < 8; ++i) ws.h[i] = sm.h[i];
     ws.plen = sm.plen;
     ws.total = sm.total;
     ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg),
-              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, sm.tail, sm.dgw, sm.out);
+              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, sm.blocks, sm.wk, sm.dgw,
+              sm.out);
     if (lane == 0) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) sm.h[i] = ws.h[i];

@@ -64,7 +64,7 @@ constexpr uint32_t TUNE_MAX_GRID = 4096;  // equals the kernels' MAXG
 
 struct TuneParams {
   uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
-  uint32_t max_grid = 2048;     // blocks of a multi-block pass (<= MAXG)
+  uint32_t max_grid = 1024;     // blocks of a multi-block pass (<= MAXG)
   uint32_t pairs_per_thread = 1;  // target pairs per thread of a multi-block pass
 };
 
@@ -160,7 +160,7 @@ class ServeReq {
 // Outputs of a finished request, kept on the host until polled.
 struct ServeResult {
   std::vector<uint8_t> msgs, point, value, final_tables;
-  zs_sha256_state sha;
+  std::vector<uint8_t> tr_state;  // the transcript stream state (zs_transcript after its header)
 };
 
 // Requests of one kind (field pair, factor count, mode): admission, one

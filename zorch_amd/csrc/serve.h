@@ -208,7 +208,7 @@ class ServeKindImpl final : public ServeKind {
     out->point.assign(h + o_point(n), h + o_point(n) + (size_t)n * sizeof(TC));
     out->value.assign((const uint8_t*)&c.claim, (const uint8_t*)&c.claim + sizeof(TC));
     out->final_tables.assign(h + o_final(n), h + o_final(n) + (K + (EQ ? 1 : 0)) * sizeof(TC));
-    memcpy(&out->sha, &c.sha, sizeof(Sha256State));
+    out->tr_state.assign((const uint8_t*)&c.tr, (const uint8_t*)&c.tr + sizeof(TrState));
   }
 
   void release(zs_ctx* ctx, ServeReq* req) override {

@@ -238,7 +238,7 @@ int take_result(zs_server* S, std::unique_lock<std::mutex>& lk, uint64_t id, zs_
     if (out->final_tables)
       memcpy(out->final_tables, f.res.final_tables.data(), f.res.final_tables.size());
   }
-  if (t) memcpy(&t->sha, &f.res.sha, sizeof(f.res.sha));
+  if (t) memcpy(&t->sha, f.res.tr_state.data(), f.res.tr_state.size());
   if (latency) *latency = f.latency;
   return ZS_OK;
 }

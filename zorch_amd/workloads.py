@@ -38,8 +38,8 @@ from zorch_amd.fields import (
 )
 
 
-def label(seed: int = 0) -> bytes:
-    return np.random.default_rng(seed).bytes(32)
+def label(seed: int = 0, nbytes: int = 32) -> bytes:
+    return np.random.default_rng(seed).bytes(nbytes)
 
 
 def table(seed: int, n: int, field: Field) -> FieldArray:
@@ -125,10 +125,20 @@ class Instance:
     label: bytes
     degree: int = 2           # ProductSummand degree (= number of tables)
     z: FieldArray | None = None
+    transcript_kind: str = "sha256"  # or "blake3"
 
     @property
     def table_elems(self) -> int:
         return self.tables.shape[0] * self.tables.shape[1]
+
+
+def new_transcript(inst: Instance):
+    """The instance's initial field transcript (its hash, label and word field)."""
+    if inst.transcript_kind == "blake3":
+        from zorch_amd.blake3_field_transcript import Blake3FieldTranscript
+        return Blake3FieldTranscript.new(inst.label, inst.transcript_field)
+    from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+    return Sha256FieldTranscript.new(inst.label, inst.transcript_field)
 
 
 def product_bb(n: int = 16, seeds=(1, 2), label_seed: int = 0) -> Instance:
