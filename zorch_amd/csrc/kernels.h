@@ -16,8 +16,9 @@
 // reduces the words mod p and finishes the round on one wavefront, one node
 // per lane: it extends t to node K+1 in eq mode (a degree-K polynomial is
 // fixed by K+1 values, so t(K+1) = sum_i (-1)^(K-i) C(K+1, i) t(i) exactly),
-// forms the message s(t) = l(t) t(t) (eq_poly.py:100-111), runs the SHA-256
-// Fiat-Shamir hop (hop.h), binds the challenge (round.py:41-49) and, where the
+// forms the message s(t) = l(t) t(t) (eq_poly.py:100-111), runs the
+// Fiat-Shamir hop of the proof's transcript (SHA-256: hop.h, BLAKE3:
+// blake3.h), binds the challenge (round.py:41-49) and, where the
 // claim is read, reduces it by Lagrange interpolation (reduce.py:27-52,
 // poly/univariate.py:36-65). The challenge stays in device memory, so a whole
 // stage is a chain of launches on one stream with no host round trip.
