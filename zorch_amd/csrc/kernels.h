@@ -52,7 +52,7 @@ namespace zs {
 constexpr int MAXPTS = 6;     // degree <= 4 product + eq factor
 constexpr int MAXK = 4;
 constexpr int PASS_THREADS = 256;
-constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_THREADS = 512;
 constexpr int MAXG = 4096;    // max blocks of one pass
 constexpr int NSLOT = 32;     // slot rows of the cross-block node sums
 constexpr int SLOTW = 32;     // 64-bit words per slot row (>= (K+1) x words/element)
@@ -463,22 +463,69 @@ __device__ __forceinline__ uint64_t vreg_el<GL>(uint64_t v) {
   return ((uint64_t)vreg((uint32_t)(v >> 32)) << 32) | vreg((uint32_t)v);
 }
 
-// LDS of the finisher: SHA-256 scratch, and the proof state between rounds
-// (stream midstate and counts, pending bytes, claim, challenge, eq mass), so
-// that only the hop's own working set occupies registers.
-template <class FC>
-struct FinishLds {
+// LDS of the Fiat-Shamir hop: SHA-256 scratch and streaming state (midstate,
+// pending bytes, length), the BLAKE3 streaming state, and the squeezed bytes.
+struct HopLds {
   alignas(16) uint8_t pend[64];
   alignas(16) uint8_t blocks[HOP_MAX_BLOCKS * 64];
   alignas(16) uint32_t wk[HOP_MAX_BLOCKS * 64];
-  alignas(16) typename FC::T msg[MAXPTS];
-  alignas(16) uint8_t buf[MAXPTS * 16 + 32];
+  alignas(16) uint8_t buf[HOP_MAX_MSG + 32];
   alignas(16) uint8_t out[64];
   uint32_t dgw[8];
   uint32_t h[8];
   uint32_t plen;
   uint64_t total;
   Blake3State b3;  // the whole stream state of a BLAKE3 transcript
+};
+
+#define ZS_LDS __attribute__((address_space(3)))
+
+// observe(msg) + sample(limbs) on the transcript state in `s`; the squeezed
+// bytes land in s.out. Wave 0, all lanes.
+__device__ __forceinline__ void fs_hop_body(HopLds& s, const uint8_t* m, uint32_t msg_len,
+                                            uint32_t item, uint32_t limbs, uint32_t kind) {
+  if (kind == TR_BLAKE3) {
+    // Every lane runs the same byte-serial stream code on the LDS state.
+    b3_fs_hop(s.b3, m, msg_len, item, limbs, s.out);
+    wave_sync();
+    return;
+  }
+  WaveSha ws;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) ws.h[i] = s.h[i];
+  ws.plen = s.plen;
+  ws.total = s.total;
+  ws_fs_hop(ws, s.pend, s.buf, m, msg_len, item, limbs, s.blocks, s.wk, s.dgw, s.out);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s.h[i] = ws.h[i];
+    s.plen = ws.plen;
+    s.total = ws.total;
+  }
+  wave_sync();
+}
+
+// The hop out of line, for the pass and batch kernels: every such kernel of a
+// translation unit calls this one copy of the two hashes' code (inlined into
+// each of them it multiplied the code size and build time many times), and
+// the LDS-typed arguments keep its accesses on LDS instructions. Callers all
+// share one occupancy target, which the function inherits. A call waits for
+// every outstanding memory operation, so the single-block tail kernel, which
+// overlaps loads of the next round with the hop, inlines the body instead.
+template <int GROUP>
+static __device__ __attribute__((noinline)) void fs_hop(ZS_LDS HopLds* hl,
+                                                        const ZS_LDS uint8_t* msg,
+                                                        uint32_t msg_len, uint32_t item,
+                                                        uint32_t limbs, uint32_t kind) {
+  fs_hop_body(*(HopLds*)hl, (const uint8_t*)msg, msg_len, item, limbs, kind);
+}
+
+// LDS of the finisher: the hop's state and scratch, and the proof state
+// between rounds (claim, challenge, eq mass), so that only the hop's own
+// working set occupies registers.
+template <class FC>
+struct FinishLds : HopLds {
+  alignas(16) typename FC::T msg[MAXPTS];
   typename FC::T claim, r, eq_prev;
 };
 
@@ -538,7 +585,7 @@ __device__ __forceinline__ void fin_store(const FinishLds<FC>& sm, Ctl<FC>* ctl)
 // the wave agree): message from the node sums, Fiat-Shamir hop, bind, eq-mass
 // update, the claim reduction when `need_claim`, and (last round) the final
 // fold. The proof state is read from and written back to `sm`.
-template <class FS, class FC, int K, bool EQ>
+template <class FS, class FC, int K, bool EQ, bool INLINE_HOP>
 __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
                                              bool need_claim, const typename FC::T* sums,
                                              FinishLds<FC>& sm,
@@ -577,26 +624,13 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   }
   wave_sync();
   if (lane == 0) trace_mark(a.trace, round, 4);
-  if (ctl->tr_kind == TR_BLAKE3) {
-    // Every lane runs the same byte-serial stream code on the LDS state.
-    b3_fs_hop(sm.b3, reinterpret_cast<const uint8_t*>(sm.msg), (size_t)(NPTS * EB),
-              ctl->item_bytes, ctl->limbs, sm.out);
-    wave_sync();
+  static_assert(NPTS * EB + 32 <= (int)sizeof(HopLds::buf), "hop framing buffer");
+  if constexpr (INLINE_HOP) {
+    fs_hop_body(sm, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
+                ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   } else {
-    WaveSha ws;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) ws.h[i] = sm.h[i];
-    ws.plen = sm.plen;
-    ws.total = sm.total;
-    ws_fs_hop(ws, sm.pend, sm.buf, reinterpret_cast<const uint8_t*>(sm.msg),
-              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, sm.blocks, sm.wk, sm.dgw,
-              sm.out);
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) sm.h[i] = ws.h[i];
-      sm.plen = ws.plen;
-      sm.total = ws.total;
-    }
+    fs_hop<0>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
+              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   }
   if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
@@ -734,8 +768,8 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
     fin_load(sh.sm, a.ctl);
   }
   if (threadIdx.x < 64) {
-    finish_round<FT, FC, K, EQ>(a, round, need_claim, sh.out, sh.sm, a.work, a.in,
-                                round == 0 ? a.in_stride : a.work_stride, round == 0);
+    finish_round<FT, FC, K, EQ, false>(a, round, need_claim, sh.out, sh.sm, a.work, a.in,
+                                   round == 0 ? a.in_stride : a.work_stride, round == 0);
     fin_store(sh.sm, a.ctl);
   }
   if (nblk > 1 && threadIdx.x == 0)
@@ -761,53 +795,241 @@ template <class FC, int NA, int THREADS>
 struct TailLds {
   typename FC::T lds[(THREADS / 64) * NA];
   typename FC::T out[NA];
+  typename FC::T lastv[MAXK][2];  // the last round's folded pair, for the final fold
   FinishLds<FC> sm;
 };
 
+// ------------------------------------------------- one-pair-per-thread rounds
+// A round r >= 2 with at most one pair per thread: thread j folds and
+// evaluates pair j alone. Its sources are table q's entries
+// T[j], T[j + L/2], T[j + L], T[j + L/2 + L] of the previous round's tables
+// (length 2L, in place in `work`), and its eq weight is wa * wb.
+template <class FC, int K>
+struct PairSrc {
+  typename FC::T s[K][4];
+  typename FC::T wa, wb;
+};
+
+template <class F>
+__device__ __forceinline__ typename F::T shfl_el(typename F::T v, int src);
+template <>
+__device__ __forceinline__ uint32_t shfl_el<BB>(uint32_t v, int src) {
+  return __shfl(v, src, 64);
+}
+template <>
+__device__ __forceinline__ bb4 shfl_el<BB4>(bb4 v, int src) {
+  return bb4_make(__shfl(v.c[0], src, 64), __shfl(v.c[1], src, 64), __shfl(v.c[2], src, 64),
+                  __shfl(v.c[3], src, 64));
+}
+template <>
+__device__ __forceinline__ uint64_t shfl_el<GL>(uint64_t v, int src) {
+  uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <class FS, class FC, int K, bool EQ>
+__device__ __forceinline__ void pair_weight_load(const StageArgs<FS, FC>& a, uint32_t round,
+                                                 uint32_t j, PairSrc<FC, K>& ps) {
+  if constexpr (EQ) {
+    const typename FC::T* wl = nullptr;
+    const typename FC::T* wr = nullptr;
+    uint32_t nxr_log = 0;
+    select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
+    if (wl) {
+      ps.wa = wl[j >> nxr_log];
+      ps.wb = wr[j & ((1u << nxr_log) - 1u)];
+    } else {
+      ps.wa = wr[j];
+      ps.wb = FC::one();
+    }
+  }
+}
+
+template <class FS, class FC, int K, bool EQ>
+__device__ __forceinline__ void pair_load(const StageArgs<FS, FC>& a, uint32_t round, uint32_t j,
+                                          PairSrc<FC, K>& ps) {
+  const size_t L = (size_t)1 << (a.n - round), half = L >> 1;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const typename FC::T* s = a.work + (size_t)q * a.work_stride;
+    ps.s[q][0] = s[j];
+    ps.s[q][1] = s[j + half];
+    ps.s[q][2] = s[j + L];
+    ps.s[q][3] = s[j + half + L];
+  }
+  pair_weight_load<FS, FC, K, EQ>(a, round, j, ps);
+}
+
+// Fold pair j's sources at the previous challenge into av = T'[j] and
+// bv = T'[j + L/2] (stored to `work` as well), and add the pair's node values
+// to acc: weight-first in eq mode, as in pass_accumulate.
+template <class FS, class FC, int K, bool EQ>
+__device__ __forceinline__ void pair_fold_eval(const StageArgs<FS, FC>& a, uint32_t round,
+                                               uint32_t j, const PairSrc<FC, K>& ps,
+                                               const ChalMul<FC>& cm, typename FC::T (&av)[K],
+                                               typename FC::T (&bv)[K],
+                                               typename FC::T (&acc)[K + 1]) {
+  using T = typename FC::T;
+  const size_t half = (size_t)1 << (a.n - round - 1);
+  T lo[K], df[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const T x = FC::add(ps.s[q][0], cm.mul(FC::sub(ps.s[q][2], ps.s[q][0])));
+    const T y = FC::add(ps.s[q][1], cm.mul(FC::sub(ps.s[q][3], ps.s[q][1])));
+    T* d = a.work + (size_t)q * a.work_stride;
+    d[j] = x;
+    d[j + half] = y;
+    av[q] = x;
+    bv[q] = y;
+    lo[q] = x;
+    df[q] = FC::sub(y, x);
+  }
+  if constexpr (EQ) {
+    const T w = FC::mul(ps.wa, ps.wb);
+    const T hi0 = FC::add(lo[0], df[0]);
+    lo[0] = FC::mul(w, lo[0]);
+    df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
+  }
+#pragma unroll
+  for (int t = 0; t <= K; ++t) {
+    T prod = lo[0];
+#pragma unroll
+    for (int q = 1; q < K; ++q) prod = FC::mul(prod, lo[q]);
+    acc[t] = FC::add(acc[t], prod);
+    if (t < K) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) lo[q] = FC::add(lo[q], df[q]);
+    }
+  }
+}
+
 // The small tail rounds [first, last) of one proof in one block of THREADS
-// threads: each round is a block-stride pass, a block reduction and the
-// finisher, separated by barriers. Global writes of one round are visible to
-// the block in the next; the transcript, challenge and eq mass stay in LDS
-// until the end.
+// threads, in three modes:
+//  * rounds that read the input tables or have more pairs than threads are
+//    block-stride passes;
+//  * later rounds with more than 64 pairs give each thread at most one pair;
+//    in the tail kernel (whose hop is inlined) the next round's sources,
+//    written by this round and visible after the reduction's barriers, are
+//    loaded into registers while wave 0 runs the hop;
+//  * once a round has at most 64 pairs, wave 0 runs the remaining rounds
+//    alone: each lane keeps its folded pair in registers, the next round's
+//    sources come from a lane shuffle, the node sums from a wave reduction,
+//    and no block barrier is taken (the other waves wait at the end).
+// One finisher call site serves all three. The transcript, challenge and eq
+// mass stay in LDS until the end.
 template <class FS, class FC, int K, bool EQ, int THREADS>
 __device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t first,
                                             uint32_t last, TailLds<FC, K + 1, THREADS>& sh) {
   using T = typename FC::T;
   constexpr int NA = K + 1;
-  if (threadIdx.x < 64) fin_load(sh.sm, a.ctl);
+  constexpr bool INLINE_HOP = THREADS == TAIL_THREADS;  // the tail kernel's own copy
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  if (tid < 64) fin_load(sh.sm, a.ctl);
   __syncthreads();
+  PairSrc<FC, K> ps;
+  T av[K], bv[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) av[q] = bv[q] = FC::zero();
+  bool have_src = false;  // ps holds this round's sources (prefetched)
+  bool from_regs = false; // this round's sources are the previous round's folded pairs
   for (uint32_t round = first; round < last; ++round) {
-    const uint32_t L = 1u << (a.n - round);
+    const uint32_t P = 1u << (a.n - round - 1);
+    const bool stride_pass = round < 2 || P > (uint32_t)THREADS;
+    const bool wave_mode = !stride_pass && P <= 64u;
+    // Wave mode lasts to the end: the other waves go straight to the final barrier.
+    if (wave_mode && tid >= 64) break;
+    ChalMul<FC> cm;
+    if (round > 0) cm.init(sh.sm.r);
     T acc[NA];
 #pragma unroll
     for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
-    const T* wl = nullptr;
-    const T* wr = nullptr;
-    uint32_t nxr_log = 0;
-    if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
-    ChalMul<FC> cm;
-    if (round > 0) cm.init(sh.sm.r);
-    if (threadIdx.x == 0) trace_mark(a.trace, round, 1);
-    if (round == 0) {
-      pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
-                                            wr, nxr_log, threadIdx.x, THREADS, acc);
-    } else if (round == 1) {
-      pass_accumulate<FS, FC, K, EQ, true>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
-                                           wr, nxr_log, threadIdx.x, THREADS, acc);
+    if (tid == 0) trace_mark(a.trace, round, 1);
+    if (stride_pass) {
+      const uint32_t L = 1u << (a.n - round);
+      const T* wl = nullptr;
+      const T* wr = nullptr;
+      uint32_t nxr_log = 0;
+      if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl, wr, nxr_log);
+      if (round == 0) {
+        pass_accumulate<FS, FC, K, EQ, false>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
+                                              wr, nxr_log, tid, THREADS, acc);
+      } else if (round == 1) {
+        pass_accumulate<FS, FC, K, EQ, true>(a.in, a.in_stride, a.work, a.work_stride, L, cm, wl,
+                                             wr, nxr_log, tid, THREADS, acc);
+      } else {
+        pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride, L, cm,
+                                             wl, wr, nxr_log, tid, THREADS, acc);
+      }
     } else {
-      pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride, L, cm,
-                                           wl, wr, nxr_log, threadIdx.x, THREADS, acc);
+      if (from_regs) {
+        // T'[j], T'[j + L/2] from lane j; T'[j + P], T'[j + P + L/2] from lane j + P.
+        const int src = (int)((lane + P) & 63u);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          ps.s[q][0] = av[q];
+          ps.s[q][1] = shfl_el<FC>(av[q], src);
+          ps.s[q][2] = bv[q];
+          ps.s[q][3] = shfl_el<FC>(bv[q], src);
+        }
+      } else if (!have_src && tid < P) {
+        pair_load<FS, FC, K, EQ>(a, round, tid, ps);
+      }
+      if (tid < P) pair_fold_eval<FS, FC, K, EQ>(a, round, tid, ps, cm, av, bv, acc);
     }
-    if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
-    block_reduce<FC, NA, THREADS>(acc, sh.lds, sh.out);
-    if (threadIdx.x == 0) trace_mark(a.trace, round, 3);
-    if (threadIdx.x < 64) {
-      finish_round<FS, FC, K, EQ>(a, round, round + 1 == last, sh.out, sh.sm, a.work, a.in,
-                                  round == 0 ? a.in_stride : a.work_stride, round == 0);
+    if (tid == 0) trace_mark(a.trace, round, 2);
+    if (wave_mode) {
+#pragma unroll
+      for (int t = 0; t < NA; ++t) {
+        T v = acc[t];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v = FC::add(v, shfl_xor_el<FC>(v, m));
+        acc[t] = v;
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < NA; ++t) sh.out[t] = acc[t];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          sh.lastv[q][0] = av[q];
+          sh.lastv[q][1] = bv[q];
+        }
+      }
+      wave_sync();
+    } else {
+      // The folded entries are complete before the reduction's barriers.
+      __builtin_amdgcn_s_waitcnt(0);
+      block_reduce<FC, NA, THREADS>(acc, sh.lds, sh.out);
     }
-    __syncthreads();
+    if (tid == 0) trace_mark(a.trace, round, 3);
+    // Next round's inputs: a wave-mode round shuffles its sources out of this
+    // round's pairs and needs only its weights; a one-pair round after a
+    // block round loads its sources now (the tail kernel) or at its start.
+    have_src = false;
+    from_regs = wave_mode;
+    if (round + 1 < last) {
+      const uint32_t Pn = P >> 1;
+      const bool next_stride = round + 1 < 2 || Pn > (uint32_t)THREADS;
+      if (!next_stride && tid < Pn) {
+        if (wave_mode) {
+          pair_weight_load<FS, FC, K, EQ>(a, round + 1, tid, ps);
+        } else if (INLINE_HOP) {
+          pair_load<FS, FC, K, EQ>(a, round + 1, tid, ps);
+          have_src = true;
+        }
+      }
+    }
+    if (tid < 64) {
+      // The final fold reads the last pair from LDS in wave mode.
+      const T* last_src = wave_mode ? &sh.lastv[0][0] : a.work;
+      const size_t last_stride = wave_mode ? 2 : round == 0 ? a.in_stride : a.work_stride;
+      finish_round<FS, FC, K, EQ, INLINE_HOP>(a, round, round + 1 == last, sh.out, sh.sm,
+                                              last_src, a.in, last_stride,
+                                              !wave_mode && round == 0);
+    }
+    if (!wave_mode) __syncthreads();
   }
-  if (threadIdx.x < 64) fin_store(sh.sm, a.ctl);
+  __syncthreads();
+  if (tid < 64) fin_store(sh.sm, a.ctl);
 }
 
 // Single-block kernel for the tail rounds of one proof.
@@ -848,6 +1070,18 @@ union BatchLds {
   TailLds<FC, NA, PASS_THREADS> tail;
 };
 
+// A tail item of a batch: all remaining rounds of one proof in this block.
+// Out of line, so that the tail's registers do not weigh on the streaming
+// loop of the pass items.
+template <class FT, class FC, int K, bool EQ>
+static __device__ __attribute__((noinline)) void batch_tail(
+    const StageArgs<FT, FC>* __restrict__ args, uint32_t round,
+    ZS_LDS TailLds<FC, K + 1, PASS_THREADS>* sh) {
+  const StageArgs<FT, FC> a = *args;
+  tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, a.n,
+                                           *(TailLds<FC, K + 1, PASS_THREADS>*)sh);
+}
+
 // RT: the round type of every item of the launch: 0 = round 0 (input tables,
 // no fold), 1 = round 1 (fold of the input tables), 2 = later rounds (fold in
 // the challenge field); 3 = mixed, chosen per item (proofs that joined a
@@ -862,7 +1096,8 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
   const StageArgs<FT, FC> a = args[it.inst];
   const uint32_t round = it.round_tail & ~TAIL_FLAG;
   if (it.round_tail & TAIL_FLAG) {
-    tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, a.n, sh.tail);
+    batch_tail<FT, FC, K, EQ>(args + it.inst, round,
+                              (ZS_LDS TailLds<FC, K + 1, PASS_THREADS>*)&sh.tail);
     return;
   }
   const uint32_t blk = blockIdx.x - it.block_off;
