@@ -154,7 +154,8 @@ int zs_ctx_profile_read(zs_ctx* ctx, zs_kernel_record* out, uint32_t cap, uint32
  * (round << 8 | phase, device wall-clock ticks) pairs; read copies up to `cap`
  * pairs, reports the wall-clock rate in kHz and clears the trace. Phases:
  * 1 pass start, 2 partial sums done, 3 block reduction done, 4 message ready,
- * 5 Fiat-Shamir hop done, 6 round bound. */
+ * 5 Fiat-Shamir hop done, 6 round bound; a chained launch adds 7 kernel entry
+ * and 8 proof state loaded (with its first round's index). */
 int zs_ctx_trace(zs_ctx* ctx, int enable);
 int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
                       uint64_t* clock_khz);
@@ -163,13 +164,18 @@ int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
  * Tuning knobs with no counterpart in the reference (XLA picks its own
  * launch shapes): rounds with at most tail_pairs pairs run in one single-block
  * kernel; a multi-block pass uses a power-of-two grid of at most max_grid
- * blocks (<= 4096) sized for pairs_per_thread pairs per thread. Results do not
- * depend on them. Set applies to proofs started afterwards. */
+ * blocks (<= 4096) sized for pairs_per_thread pairs per thread; in zs_prove
+ * and the per-round calls, rounds from the third on with at most chain_pairs
+ * pairs (and more than tail_pairs) run in one persistent launch of at most
+ * chain_grid blocks (power of two; chain_pairs 0: one launch per round).
+ * Results do not depend on them. Set applies to proofs started afterwards. */
 typedef struct zs_tune {
   uint32_t tail_pairs;
   uint32_t max_grid;
   uint32_t pairs_per_thread;
-  uint32_t reserved;
+  uint32_t chain_pairs;
+  uint32_t chain_grid;
+  uint32_t reserved[3];
 } zs_tune;
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out);
 int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* tune);

@@ -156,6 +156,52 @@ def test_product_bb_n16_full_size_and_determinism():
     assert a.reduction_proof.words.tolist() == b.reduction_proof.words.tolist()
 
 
+def _prove_with_tune(inst, **tune):
+    from zorch_amd.runtime import default_context
+    ctx = default_context()
+    old = ctx.get_tune()
+    try:
+        ctx.set_tune(**tune)
+        return _prove(inst)
+    finally:
+        ctx.set_tune(**old)
+
+
+@pytest.mark.parametrize("tune", [
+    dict(chain_pairs=1 << 20, max_grid=8),                   # few blocks, many pairs each
+    dict(chain_pairs=1 << 20, tail_pairs=0),                 # chain runs to the last round
+    dict(chain_pairs=1 << 20, tail_pairs=64, pairs_per_thread=4, max_grid=4096),
+    dict(chain_pairs=1 << 12, tail_pairs=256, max_grid=64),
+])
+def test_chained_rounds_equal_one_launch_per_round(tune):
+    """Rounds run in one persistent launch equal one launch per round, bit
+    for bit, whatever the launch shapes; the C oracle anchors one of them."""
+    insts = [W.zerocheck_bb4(17, (91, 92), 93, 11), W.product_gl(16, (94, 95), 12),
+             W.product_bb(15, (96, 97, 98), 13), W.zerocheck_bb4(16, (99, 100), 101, 14)]
+    insts[3].transcript_kind = "blake3"
+    for inst in insts:
+        ref = _prove_with_tune(inst, chain_pairs=0)
+        got = _prove_with_tune(inst, **tune)
+        assert got.reduction_proof.words.tolist() == ref.reduction_proof.words.tolist()
+        assert got.reduced_claim.point.words.tolist() == ref.reduced_claim.point.words.tolist()
+        assert got.reduced_claim.value.words.tolist() == ref.reduced_claim.value.words.tolist()
+        assert got.transcript.stream_state() == ref.transcript.stream_state()
+    _compare_with_c(insts[0])
+
+
+def test_chained_rounds_round_by_round():
+    """The per-round drop-in path runs chained single rounds; it equals the
+    whole stage."""
+    inst = W.zerocheck_bb4(17, (102, 103), 104, 15)
+    whole = _prove(inst)
+    t0 = W.new_transcript(inst)
+    rnd = EqPolyRound(ProductSummand(inst.degree), inst.z, challenges=ChallengePolicy(inst.chal))
+    carry, t, msgs = fold_rounds(rnd, initial_claim(inst.tables, inst.claim, inst.n), t0, inst.n)
+    assert [m.words.tolist() for m in msgs] == whole.reduction_proof.words.tolist()
+    assert carry.claim.value.words.tolist() == whole.reduced_claim.value.words.tolist()
+    assert t.stream_state() == whole.transcript.stream_state()
+
+
 def test_device_resident_tables_equal_host_tables():
     from zorch_amd.runtime import DeviceTables
     inst = W.zerocheck_bb4(14, (81, 82), 83, 9)

@@ -102,7 +102,9 @@ class TuneC(ctypes.Structure):
         ("tail_pairs", ctypes.c_uint32),
         ("max_grid", ctypes.c_uint32),
         ("pairs_per_thread", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("chain_pairs", ctypes.c_uint32),
+        ("chain_grid", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 

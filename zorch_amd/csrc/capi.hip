@@ -338,7 +338,8 @@ int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out) {
   if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
-  *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread, 0};
+  *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread,
+                 ctx->tune.chain_pairs, ctx->tune.chain_grid, {0, 0, 0}};
   return ZS_OK;
 }
 
@@ -349,9 +350,14 @@ int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* t) {
     return fail(ZS_EINVAL, "max_grid must be a power of two in [1, 4096]");
   if (t->pairs_per_thread == 0 || t->pairs_per_thread > (1u << 20))
     return fail(ZS_EINVAL, "pairs_per_thread must be in [1, 2^20]");
+  const uint32_t cg = t->chain_grid;
+  if (cg == 0 || cg > zs::TUNE_MAX_GRID || (cg & (cg - 1)))
+    return fail(ZS_EINVAL, "chain_grid must be a power of two in [1, 4096]");
   ctx->tune.tail_pairs = t->tail_pairs;
   ctx->tune.max_grid = g;
   ctx->tune.pairs_per_thread = t->pairs_per_thread;
+  ctx->tune.chain_pairs = t->chain_pairs;
+  ctx->tune.chain_grid = t->chain_grid;
   return ZS_OK;
 }
 

@@ -198,6 +198,15 @@ class InstanceImpl final : public Instance {
         }
         return ZS_OK;
       }
+      if (r >= 2 && pairs <= tune_.chain_pairs) {
+        // Chained: rounds r .. (tail or last) in one persistent launch.
+        uint32_t end = r;
+        while (end < last && ((size_t)1 << (n_ - end - 1)) > tune_.tail_pairs) ++end;
+        int rc = enqueue_chain(r, end, last, s);
+        if (rc) return rc;
+        r = end - 1;
+        continue;
+      }
       size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
       size_t grid = (pairs + per_block - 1) / per_block;
       uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
@@ -224,6 +233,43 @@ class InstanceImpl final : public Instance {
     return ZS_OK;
   }
 
+  // Rounds [first, end) in one launch of chain_kernel: one block per
+  // PASS_THREADS * pairs_per_thread pairs of the first round, a power of two,
+  // at most max_grid, chain_grid and half the blocks the device holds at
+  // once, so that every block is resident even beside other work (blocks wait
+  // for one another; their waits are bounded, so a violation faults the proof
+  // instead of hanging the device).
+  int enqueue_chain(uint32_t first, uint32_t end, uint32_t last, hipStream_t s) {
+    static int resident = 0;  // per instantiation: blocks of this kernel the device holds
+    auto kern = chain_kernel<FT, FC, K, EQ>;
+    if (resident == 0) {
+      int per_cu = 0, dev = 0, cus = 0;
+      ZS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PASS_THREADS, 0));
+      ZS_HIP(hipGetDevice(&dev));
+      ZS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      resident = per_cu * cus;
+      if (resident < 1) return fail(ZS_EHIP, "chained rounds: kernel does not fit on the device");
+    }
+    const size_t pairs = (size_t)1 << (n_ - first - 1);
+    const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
+    size_t grid = (pairs + per_block - 1) / per_block;
+    uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
+    if (grid > maxg) grid = maxg;
+    if (grid > (size_t)resident / 2) grid = (size_t)resident / 2;
+    if (grid > tune_.chain_grid) grid = tune_.chain_grid;
+    if (grid < 1) grid = 1;
+    while (grid & (grid - 1)) grid &= grid - 1;
+    hipEvent_t ev = begin(s);
+    kern<<<(uint32_t)grid, PASS_THREADS, 0, s>>>(args_, first, end, last);
+    ZS_HIP(hipGetLastError());
+    if (ev) {
+      uint64_t b = 0;
+      for (uint32_t q = first; q < end; ++q) b += pass_bytes(q);
+      mark(s, first, KREC_CHAIN, (uint32_t)grid, b, ev);
+    }
+    return ZS_OK;
+  }
+
   int download(hipStream_t s) override {
     ZS_HIP(hipMemcpyAsync(host_, block_, out_bytes_, hipMemcpyDeviceToHost, s));
     return ZS_OK;
@@ -232,6 +278,7 @@ class InstanceImpl final : public Instance {
   int collect(zs_proof* out, zs_transcript* t) override {
     const uint8_t* h = (const uint8_t*)host_;
     const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
+    if (c.fault) return fail(ZS_EHIP, "chained rounds: a device wait reached its limit");
     if (out->msgs) memcpy(out->msgs, h + o_msgs_, (size_t)n_ * NPTS * sizeof(TC));
     if (out->point) memcpy(out->point, h + o_point_, (size_t)n_ * sizeof(TC));
     if (out->value) memcpy(out->value, &c.claim, sizeof(TC));
@@ -258,6 +305,7 @@ class InstanceImpl final : public Instance {
   int collect_round(uint32_t round, void* msg, void* chal, void* claim, zs_transcript* t) override {
     const uint8_t* h = (const uint8_t*)host_;
     const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
+    if (c.fault) return fail(ZS_EHIP, "chained rounds: a device wait reached its limit");
     if (msg) memcpy(msg, h + o_msgs_ + (size_t)round * NPTS * sizeof(TC), NPTS * sizeof(TC));
     if (chal) memcpy(chal, h + o_point_ + (size_t)round * sizeof(TC), sizeof(TC));
     if (claim) memcpy(claim, &c.claim, sizeof(TC));

@@ -71,7 +71,10 @@ struct Ctl {
   uint32_t tr_kind;      // TR_SHA256 or TR_BLAKE3
   uint32_t item_bytes;   // transcript word size
   uint32_t limbs;        // transcript words per challenge
-  uint32_t counter;      // last-block ticket of the current pass
+  uint32_t counter;      // last-block ticket of the current pass; chained arrivals
+  uint32_t done;         // rounds published by a chained launch
+  uint32_t fault;        // a chained wait reached its limit
+  uint32_t pad_;
   typename FC::T claim;
   typename FC::T r;        // current challenge, canonical
   typename FC::T eq_prev;  // eq mass of the bound prefix (eq mode)
@@ -512,6 +515,9 @@ __device__ __forceinline__ void fs_hop_body(HopLds& s, const uint8_t* m, uint32_
 // share one occupancy target, which the function inherits. A call waits for
 // every outstanding memory operation, so the single-block tail kernel, which
 // overlaps loads of the next round with the hop, inlines the body instead.
+// Hop variants of finish_round: inlined, or the out-of-line copy of a group.
+constexpr int HOP_INLINE = -1, HOP_PASS = 0, HOP_CHAIN = 1;
+
 template <int GROUP>
 static __device__ __attribute__((noinline)) void fs_hop(ZS_LDS HopLds* hl,
                                                         const ZS_LDS uint8_t* msg,
@@ -585,7 +591,7 @@ __device__ __forceinline__ void fin_store(const FinishLds<FC>& sm, Ctl<FC>* ctl)
 // the wave agree): message from the node sums, Fiat-Shamir hop, bind, eq-mass
 // update, the claim reduction when `need_claim`, and (last round) the final
 // fold. The proof state is read from and written back to `sm`.
-template <class FS, class FC, int K, bool EQ, bool INLINE_HOP>
+template <class FS, class FC, int K, bool EQ, int HOP>
 __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_t round,
                                              bool need_claim, const typename FC::T* sums,
                                              FinishLds<FC>& sm,
@@ -625,12 +631,12 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   wave_sync();
   if (lane == 0) trace_mark(a.trace, round, 4);
   static_assert(NPTS * EB + 32 <= (int)sizeof(HopLds::buf), "hop framing buffer");
-  if constexpr (INLINE_HOP) {
+  if constexpr (HOP == HOP_INLINE) {
     fs_hop_body(sm, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
                 ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   } else {
-    fs_hop<0>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
-              (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    fs_hop<HOP>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
+                (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   }
   if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
@@ -686,6 +692,61 @@ struct RoundLds {
   uint32_t is_last;
 };
 
+// Add a block's node sums (NA elements in LDS), word by word, to slot row
+// blk % NSLOT.
+template <class FC, int NA>
+__device__ __forceinline__ void slot_row_add(uint64_t* slots, uint32_t blk,
+                                             const typename FC::T* out) {
+  constexpr int W = SlotOps<FC>::W;
+  constexpr int NW = NA * W;
+  static_assert(NW <= SLOTW && NSLOT % (PASS_THREADS / SLOTW) == 0, "slot row layout");
+  if (threadIdx.x < NW) {
+    uint64_t* slot = slots + (size_t)(blk & (NSLOT - 1)) * SLOTW + threadIdx.x;
+    const uint32_t word = SlotOps<FC>::word(out[threadIdx.x / W], threadIdx.x % W);
+    __hip_atomic_fetch_add(slot, (uint64_t)word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Sum of the slot rows into sh.out (NA elements), which it clears for the
+// next round; ends with a barrier. Every block that added to the rows has
+// completed its additions.
+template <class FC, int NA>
+__device__ __forceinline__ void slot_rows_sum(uint64_t* slots, RoundLds<FC, NA>& sh) {
+  constexpr int W = SlotOps<FC>::W;
+  constexpr int NW = NA * W;
+  constexpr int G = PASS_THREADS / SLOTW;
+  // Thread (g, w) sums word w of rows g, g + G, ...
+  const uint32_t w = threadIdx.x % SLOTW, g = threadIdx.x / SLOTW;
+  uint64_t s = 0;
+  if (w < (uint32_t)NW) {
+#pragma unroll
+    for (int i = 0; i < NSLOT / G; ++i)
+      s += __hip_atomic_load(slots + (size_t)(g + i * G) * SLOTW + w, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+  sh.sum64[g][w] = s;
+  __syncthreads();
+  // Every row has been read: clear them for the next round.
+  if (w < (uint32_t)NW) {
+#pragma unroll
+    for (int i = 0; i < NSLOT / G; ++i)
+      __hip_atomic_store(slots + (size_t)(g + i * G) * SLOTW + w, (uint64_t)0, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < NA) {
+    uint64_t tot[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      uint64_t v = 0;
+#pragma unroll
+      for (int q = 0; q < G; ++q) v += sh.sum64[q][threadIdx.x * W + c];
+      tot[c] = v;
+    }
+    sh.out[threadIdx.x] = SlotOps<FC>::join(tot);
+  }
+  __syncthreads();
+}
+
 // One block's share of a multi-block round: block `blk` of `nblk` accumulates
 // its pairs and adds its node sums into slot row blk % NSLOT; the last block
 // of the round to take a ticket sums the rows, clears them and finishes the
@@ -713,15 +774,7 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
                                        nxr_log, start, stride, acc);
   block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
   if (nblk > 1) {
-    constexpr int W = SlotOps<FC>::W;
-    constexpr int NW = NA * W;
-    constexpr int G = PASS_THREADS / SLOTW;
-    static_assert(NW <= SLOTW && NSLOT % G == 0, "slot row layout");
-    if (threadIdx.x < NW) {
-      uint64_t* slot = a.slots + (size_t)(blk & (NSLOT - 1)) * SLOTW + threadIdx.x;
-      const uint32_t word = SlotOps<FC>::word(sh.out[threadIdx.x / W], threadIdx.x % W);
-      __hip_atomic_fetch_add(slot, (uint64_t)word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    slot_row_add<FC, NA>(a.slots, blk, sh.out);
     // The slot additions complete before the ticket is taken.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -734,42 +787,13 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
     if (!sh.is_last) return;
     if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
     if (threadIdx.x < 64) fin_load(sh.sm, a.ctl);
-    // Thread (g, w) sums word w of rows g, g + G, ...
-    const uint32_t w = threadIdx.x % SLOTW, g = threadIdx.x / SLOTW;
-    uint64_t s = 0;
-    if (w < (uint32_t)NW) {
-#pragma unroll
-      for (int i = 0; i < NSLOT / G; ++i)
-        s += __hip_atomic_load(a.slots + (size_t)(g + i * G) * SLOTW + w, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
-    sh.sum64[g][w] = s;
-    __syncthreads();
-    // Every row has been read: clear them for the next round.
-    if (w < (uint32_t)NW) {
-#pragma unroll
-      for (int i = 0; i < NSLOT / G; ++i)
-        __hip_atomic_store(a.slots + (size_t)(g + i * G) * SLOTW + w, (uint64_t)0,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (threadIdx.x < NA) {
-      uint64_t tot[W];
-#pragma unroll
-      for (int c = 0; c < W; ++c) {
-        uint64_t v = 0;
-#pragma unroll
-        for (int q = 0; q < G; ++q) v += sh.sum64[q][threadIdx.x * W + c];
-        tot[c] = v;
-      }
-      sh.out[threadIdx.x] = SlotOps<FC>::join(tot);
-    }
-    __syncthreads();
+    slot_rows_sum<FC, NA>(a.slots, sh);
   } else if (threadIdx.x < 64) {
     fin_load(sh.sm, a.ctl);
   }
   if (threadIdx.x < 64) {
-    finish_round<FT, FC, K, EQ, false>(a, round, need_claim, sh.out, sh.sm, a.work, a.in,
-                                   round == 0 ? a.in_stride : a.work_stride, round == 0);
+    finish_round<FT, FC, K, EQ, HOP_PASS>(a, round, need_claim, sh.out, sh.sm, a.work, a.in,
+                                          round == 0 ? a.in_stride : a.work_stride, round == 0);
     fin_store(sh.sm, a.ctl);
   }
   if (nblk > 1 && threadIdx.x == 0)
@@ -788,6 +812,128 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(St
   __shared__ RoundLds<FC, K + 1> sh;
   round_block<FT, FS, FC, K, EQ, FOLD>(a, src, src_stride, L, round, need_claim != 0, blockIdx.x,
                                        gridDim.x, sh);
+}
+
+// ------------------------------------------------------------ chained rounds
+// Polls of a chained-kernel wait, each after a short sleep: far above any real
+// wait, the limit only keeps a fault from hanging the device. A wait that
+// reaches it marks the proof faulted (the host reports an error) and the
+// block exits.
+constexpr uint32_t CHAIN_SPIN_LIMIT = 1u << 22;
+
+// Wait until *p >= target, then acquire (device scope). False on a fault.
+// The polls are relaxed: an acquiring load invalidates the caches it passes
+// through, and a storm of them stalls every other wave on the device.
+__device__ __forceinline__ bool chain_wait(uint32_t* p, uint32_t target, uint32_t* fault) {
+  for (uint32_t i = 0; i < CHAIN_SPIN_LIMIT; ++i) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      return true;
+    }
+    if (__hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+// Blocks that share round `round` of a chained launch of `grid` blocks: one
+// per PASS_THREADS pairs, at most the grid. Non-increasing in the round, so
+// a block without work in a round has none later.
+__device__ __forceinline__ uint32_t chain_blocks(uint32_t n, uint32_t round, uint32_t grid) {
+  const uint32_t per = (1u << (n - round - 1)) / PASS_THREADS;
+  return per < 1u ? 1u : per < grid ? per : grid;
+}
+
+// Rounds [first, last) (all >= 2: tables in the challenge field) of one proof
+// in one persistent launch, whose grid is small enough for all its blocks to
+// be resident at once (blocks wait for one another). Per round, every block with work folds and evaluates its
+// grid-stride share, adds its node sums to the slot rows and arrives at the
+// control block's counter (release). Block 0 waits for every arrival, sums
+// and clears the rows, runs the finisher (the proof state stays in its LDS
+// across rounds), publishes the challenge and bumps `done` (release); the
+// other blocks wait for `done` (acquire) and read the challenge. Against one
+// launch per round this removes the launch gap and keeps the finisher's code
+// warm in one compute unit's instruction cache. The claim is reduced at
+// round claim_last - 1 only.
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) void chain_kernel(
+    StageArgs<FT, FC> a, uint32_t first, uint32_t last, uint32_t claim_last) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1;
+  __shared__ RoundLds<FC, NA> sh;
+  __shared__ T r_sh;
+  __shared__ uint32_t ok_sh;
+  const uint32_t blk = blockIdx.x, tid = threadIdx.x;
+  Ctl<FC>* ctl = a.ctl;
+  if (blk == 0 && tid == 0) trace_mark(a.trace, first, 7);
+  if (blk == 0 && tid < 64) fin_load(sh.sm, ctl);
+  if (tid == 0) r_sh = ctl->r;
+  __syncthreads();
+  if (blk == 0 && tid == 0) trace_mark(a.trace, first, 8);
+  uint32_t arrivals = 0;  // counter value once every block has arrived in this round
+  for (uint32_t round = first; round < last; ++round) {
+    const uint32_t nb = chain_blocks(a.n, round, gridDim.x);
+    if (blk >= nb) break;
+    if (nb > 1) arrivals += nb;
+    T acc[NA];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
+    const T* wl = nullptr;
+    const T* wr = nullptr;
+    uint32_t nxr_log = 0;
+    if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl, wr, nxr_log);
+    ChalMul<FC> cm;
+    cm.init(r_sh);
+    if (blk == 0 && tid == 0) trace_mark(a.trace, round, 1);
+    pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride,
+                                         1u << (a.n - round), cm, wl, wr, nxr_log,
+                                         (size_t)blk * PASS_THREADS + tid,
+                                         (size_t)nb * PASS_THREADS, acc);
+    block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
+    if (nb > 1) {
+      slot_row_add<FC, NA>(a.slots, blk, sh.out);
+      // Slot additions and folded entries are complete before the arrival.
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blk == 0) {
+      if (nb > 1) {
+        if (tid == 0) ok_sh = chain_wait(&ctl->counter, arrivals, &ctl->fault) ? 1u : 0u;
+        __syncthreads();
+        if (!ok_sh) return;
+        if (tid == 0) trace_mark(a.trace, round, 2);
+        slot_rows_sum<FC, NA>(a.slots, sh);
+      }
+      if (tid < 64) {
+        finish_round<FT, FC, K, EQ, HOP_CHAIN>(a, round, round + 1 == claim_last, sh.out, sh.sm,
+                                               a.work, a.in, a.work_stride, false);
+      }
+      if (tid == 0) {
+        r_sh = sh.sm.r;
+        ctl->r = sh.sm.r;
+        __hip_atomic_store(&ctl->done, round - first + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+    } else if (round + 1 < last && blk < chain_blocks(a.n, round + 1, gridDim.x)) {
+      if (tid == 0) {
+        const bool ok = chain_wait(&ctl->done, round - first + 1, &ctl->fault);
+        if (ok) r_sh = ctl->r;
+        ok_sh = ok ? 1u : 0u;
+      }
+      __syncthreads();
+      if (!ok_sh) return;
+    }
+  }
+  if (blk == 0) {
+    // Every other block has arrived for the last time and waits no more.
+    if (tid < 64) fin_store(sh.sm, ctl);
+    if (tid == 0) {
+      __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // LDS of the single-block tail loop.
@@ -922,7 +1068,7 @@ __device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t
                                             uint32_t last, TailLds<FC, K + 1, THREADS>& sh) {
   using T = typename FC::T;
   constexpr int NA = K + 1;
-  constexpr bool INLINE_HOP = THREADS == TAIL_THREADS;  // the tail kernel's own copy
+  constexpr int HOP = THREADS == TAIL_THREADS ? HOP_INLINE : HOP_PASS;
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   if (tid < 64) fin_load(sh.sm, a.ctl);
   __syncthreads();
@@ -1012,7 +1158,7 @@ __device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t
       if (!next_stride && tid < Pn) {
         if (wave_mode) {
           pair_weight_load<FS, FC, K, EQ>(a, round + 1, tid, ps);
-        } else if (INLINE_HOP) {
+        } else if (HOP == HOP_INLINE) {
           pair_load<FS, FC, K, EQ>(a, round + 1, tid, ps);
           have_src = true;
         }
@@ -1022,7 +1168,7 @@ __device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t
       // The final fold reads the last pair from LDS in wave mode.
       const T* last_src = wave_mode ? &sh.lastv[0][0] : a.work;
       const size_t last_stride = wave_mode ? 2 : round == 0 ? a.in_stride : a.work_stride;
-      finish_round<FS, FC, K, EQ, INLINE_HOP>(a, round, round + 1 == last, sh.out, sh.sm,
+      finish_round<FS, FC, K, EQ, HOP>(a, round, round + 1 == last, sh.out, sh.sm,
                                               last_src, a.in, last_stride,
                                               !wave_mode && round == 0);
     }

@@ -63,9 +63,12 @@ class HostPool {
 constexpr uint32_t TUNE_MAX_GRID = 4096;  // equals the kernels' MAXG
 
 struct TuneParams {
-  uint32_t tail_pairs = 2048;   // rounds with at most this many pairs run in the tail kernel
+  uint32_t tail_pairs = 1024;   // rounds with at most this many pairs run in the tail kernel
   uint32_t max_grid = 1024;     // blocks of a multi-block pass (<= MAXG)
   uint32_t pairs_per_thread = 1;  // target pairs per thread of a multi-block pass
+  uint32_t chain_pairs = 1u << 18;  // rounds >= 2 with at most this many pairs (and more than
+                                    // tail_pairs) run chained in one persistent launch; 0: off
+  uint32_t chain_grid = 128;        // blocks of a chained launch (at most)
 };
 
 // Kernel timing records (enabled by zs_ctx_profile).
@@ -74,7 +77,7 @@ struct KernelRecord {
   uint64_t bytes;
   hipEvent_t start, stop;
 };
-enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4 };
+enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5 };
 
 }  // namespace zs
 
