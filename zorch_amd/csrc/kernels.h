@@ -316,105 +316,145 @@ __device__ __forceinline__ void pass_accumulate(
   // The 64-bit weighted sums of BabyBear^4 weights on BabyBear values take
   // the registers of two more unrolled pairs.
   constexpr bool WIDE_SUMS = EQ && !WEIGHT_FIRST && std::is_same<FC, BB4>::value;
+  // Narrow sources (at most 32 bytes a pair) are double-buffered, 32 bytes a
+  // step: the next step's loads (tables and eq weights) are issued before this
+  // step's arithmetic, so that every wave overlaps its own loads with its
+  // compute; wider sources already fill the register budget and rely on the
+  // other waves of the SIMD.
+  constexpr int PAIR_BYTES = K * NS * (int)sizeof(S);
+  constexpr bool PREFETCH = PAIR_BYTES <= 32;
   constexpr int U0 = PassUnroll<FS, K, FOLD>::value;
-  constexpr int U = WIDE_SUMS && U0 > 2 ? 2 : U0;
+  constexpr int U = PREFETCH ? 32 / PAIR_BYTES : WIDE_SUMS && U0 > 2 ? 2 : U0;
   const size_t half = L >> 1;
   const size_t nxr_mask = ((size_t)1 << nxr_log) - 1;
   const bool factored = EQ && wl != nullptr && (stride & nxr_mask) == 0;
   WeightedSum<FC, FV> wsum[K + 1];
 #pragma unroll
   for (int t = 0; t <= K; ++t) wsum[t].init();
-  for (size_t j0 = start; j0 < half; j0 += (size_t)U * stride) {
-    S raw[U][K][NS];
+  // Eq weight of pair j: wsel[j >> wshift], times wr[j & wmask] when the
+  // eq_r factor is not applied per thread (wmask 0 reads a dummy otherwise);
+  // all uniform, so the weight loads take no branch.
+  const T* wsel = wl ? wl : wr;
+  const uint32_t wshift = wl ? nxr_log : 0u;
+  const bool wmul = EQ && wl != nullptr && !factored;
+  const size_t wmask = wmul ? nxr_mask : 0;
+  // Sources of the U pairs starting at j0 into r; indices past the round are
+  // clamped into it, so every load is issued whatever the step (an exact
+  // count of loads in flight lets each wait leave the prefetched ones alone).
+  auto load = [&](size_t j0, S (&r)[U][K][NS]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      size_t j = j0 + (size_t)u * stride;
+      j = j < half ? j : half - 1;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const S* s = src + (size_t)q * src_stride;
+        r[u][q][0] = s[j];
+        r[u][q][1] = s[j + half];
+        if constexpr (FOLD) {
+          r[u][q][2] = s[j + L];
+          r[u][q][3] = s[j + half + L];
+        }
+      }
+    }
+  };
+  // Fold (FOLD) and evaluate pair j from its sources r with weight w.
+  auto pair = [&](size_t j, const S (&r)[K][NS], T w) {
+    V lo[K], df[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      if constexpr (FOLD) {
+        const S s0 = r[q][0], s2 = r[q][1], s1 = r[q][2], s3 = r[q][3];
+        V a = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, FS::sub(s1, s0)));
+        V b = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, FS::sub(s3, s2)));
+        T* d = dst + (size_t)q * dst_stride;
+        d[j] = a;
+        d[j + half] = b;
+        lo[q] = a;
+        df[q] = FC::sub(b, a);
+      } else {
+        lo[q] = r[q][0];
+        df[q] = FV::sub(r[q][1], r[q][0]);
+      }
+    }
+    if constexpr (EQ && WEIGHT_FIRST) {
+      V hi0 = FV::add(lo[0], df[0]);
+      lo[0] = FC::mul(w, lo[0]);
+      df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
+    }
+#pragma unroll
+    for (int t = 0; t <= K; ++t) {
+      V prod = lo[0];
+#pragma unroll
+      for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
+      if constexpr (EQ && !WEIGHT_FIRST) {
+        wsum[t].add(w, prod);
+      } else {
+        acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
+      }
+      if (t < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
+      }
+    }
+    // One pair's arithmetic at a time: interleaving the unrolled pairs would
+    // only raise register pressure (their loads are already issued).
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  constexpr int UW = EQ ? U : 1;
+  const size_t step = (size_t)U * stride;
+  // Steps whose U pairs are in the round for every thread (start < stride).
+  const size_t nfull = half / step;
+  size_t j0 = start;
+  S raw[U][K][NS];
+  if (PREFETCH && nfull > 0) load(j0, raw);
+  for (size_t k = 0; k < nfull; ++k, j0 += step) {
+    // This step's weight operands, issued before the next step's sources so
+    // that waiting for them leaves those in flight.
+    T wa[UW], wb[UW];
+    if constexpr (EQ) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const size_t j = j0 + (size_t)u * stride;
+        wa[u] = wsel[j >> wshift];
+        wb[u] = wr[j & wmask];
+      }
+    }
+    S nxt[U][K][NS];
+    if constexpr (PREFETCH) {
+      load(j0 + step, nxt);
+    } else {
+      load(j0, raw);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      T w = FC::one();
+      if constexpr (EQ) w = wmul ? FC::mul(wa[u], wb[u]) : wa[u];
+      pair(j0 + (size_t)u * stride, raw[u], w);
+    }
+    if constexpr (PREFETCH) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+#pragma unroll
+          for (int c = 0; c < NS; ++c) raw[u][q][c] = nxt[u][q][c];
+    }
+  }
+  // The last, partial step: pairs j0 + u stride that are still in the round.
+  if (j0 < half) {
+    load(j0, raw);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const size_t j = j0 + (size_t)u * stride;
       if (j < half) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-          const S* s = src + (size_t)q * src_stride;
-          raw[u][q][0] = s[j];
-          raw[u][q][1] = s[j + half];
-          if constexpr (FOLD) {
-            raw[u][q][2] = s[j + L];
-            raw[u][q][3] = s[j + half + L];
-          }
+        T w = FC::one();
+        if constexpr (EQ) {
+          w = wsel[j >> wshift];
+          if (wmul) w = FC::mul(w, wr[j & wmask]);
         }
+        pair(j, raw[u], w);
       }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const size_t j = j0 + (size_t)u * stride;
-      if (j >= half) break;
-      V lo[K], df[K];
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        if constexpr (FOLD) {
-          const S s0 = raw[u][q][0], s2 = raw[u][q][1], s1 = raw[u][q][2], s3 = raw[u][q][3];
-          V a = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, FS::sub(s1, s0)));
-          V b = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, FS::sub(s3, s2)));
-          T* d = dst + (size_t)q * dst_stride;
-          d[j] = a;
-          d[j + half] = b;
-          lo[q] = a;
-          df[q] = FC::sub(b, a);
-        } else {
-          lo[q] = raw[u][q][0];
-          df[q] = FV::sub(raw[u][q][1], raw[u][q][0]);
-        }
-      }
-      if constexpr (EQ) {
-        T w;
-        if (wl) {
-          w = wl[j >> nxr_log];
-          if (!factored) w = FC::mul(w, wr[j & nxr_mask]);
-        } else {
-          w = wr[j];
-        }
-        if constexpr (WEIGHT_FIRST) {
-          V hi0 = FV::add(lo[0], df[0]);
-          lo[0] = FC::mul(w, lo[0]);
-          df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
-#pragma unroll
-          for (int t = 0; t <= K; ++t) {
-            V prod = lo[0];
-#pragma unroll
-            for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-            acc[t] = FC::add(acc[t], prod);
-            if (t < K) {
-#pragma unroll
-              for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
-            }
-          }
-        } else {
-#pragma unroll
-          for (int t = 0; t <= K; ++t) {
-            V prod = lo[0];
-#pragma unroll
-            for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-            wsum[t].add(w, prod);
-            if (t < K) {
-#pragma unroll
-              for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
-            }
-          }
-        }
-      } else {
-#pragma unroll
-        for (int t = 0; t <= K; ++t) {
-          V prod = lo[0];
-#pragma unroll
-          for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
-          acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
-          if (t < K) {
-#pragma unroll
-            for (int q = 0; q < K; ++q) lo[q] = FV::add(lo[q], df[q]);
-          }
-        }
-      }
-      // One pair's arithmetic at a time: interleaving the unrolled pairs
-      // would only raise register pressure (their loads are already issued).
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
   if constexpr (EQ && !WEIGHT_FIRST) {
@@ -450,6 +490,22 @@ __device__ __forceinline__ uint64_t el_from_bytes<GL>(const uint8_t* b) {
   uint64_t w = 0;
   for (int i = 0; i < 8; ++i) w |= (uint64_t)b[i] << (8 * i);
   return gl_canon(w);
+}
+
+// The same from 4-byte aligned little-endian words.
+template <class F>
+__device__ __forceinline__ typename F::T el_from_words(const uint32_t* w);
+template <>
+__device__ __forceinline__ uint32_t el_from_words<BB>(const uint32_t* w) {
+  return bb_canon(w[0]);
+}
+template <>
+__device__ __forceinline__ bb4 el_from_words<BB4>(const uint32_t* w) {
+  return bb4_make(bb_canon(w[0]), bb_canon(w[1]), bb_canon(w[2]), bb_canon(w[3]));
+}
+template <>
+__device__ __forceinline__ uint64_t el_from_words<GL>(const uint32_t* w) {
+  return gl_canon(((uint64_t)w[1] << 32) | w[0]);
 }
 
 // A field element in vector registers (see vreg in hop.h).
@@ -533,6 +589,10 @@ template <class FC>
 struct FinishLds : HopLds {
   alignas(16) typename FC::T msg[MAXPTS];
   typename FC::T claim, r, eq_prev;
+  // Round constants, read once per kernel: nodes, extension coefficients,
+  // Lagrange denominators; the current round's eq line coefficients.
+  typename FC::T nodes[MAXPTS], ext_coef[MAXPTS], lag_inv[MAXPTS];
+  typename FC::T eqA, eqB;
 };
 
 // Control block -> finisher LDS (wave 0, all lanes).
@@ -556,7 +616,23 @@ __device__ __forceinline__ void fin_load(FinishLds<FC>& sm, const Ctl<FC>* ctl) 
     sm.r = ctl->r;
     sm.eq_prev = ctl->eq_prev;
   }
+  if (lane < (uint32_t)MAXPTS) {
+    sm.nodes[lane] = ctl->nodes[lane];
+    sm.ext_coef[lane] = ctl->ext_coef[lane];
+    sm.lag_inv[lane] = ctl->lag_inv[lane];
+  }
   wave_sync();
+}
+
+// The eq line coefficients of `round` -> finisher LDS (wave 0, lane 0); read
+// by finish_round after the next wave or block barrier.
+template <class FS, class FC, bool EQ>
+__device__ __forceinline__ void fin_eq(FinishLds<FC>& sm, const StageArgs<FS, FC>& a,
+                                       uint32_t round) {
+  if (EQ && (threadIdx.x & 63) == 0) {
+    sm.eqA = a.eq_ab[2 * round];
+    sm.eqB = a.eq_ab[2 * round + 1];
+  }
 }
 
 // Finisher LDS -> control block (wave 0, all lanes); pending bytes past the
@@ -612,9 +688,9 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
     if (t > (uint32_t)K) tv = FC::zero();
     T msg;
     if constexpr (EQ) {
-      const T A = a.eq_ab[2 * round], B = a.eq_ab[2 * round + 1];
-      const T node = ctl->nodes[tn];
-      const T coef = ctl->ext_coef[t <= (uint32_t)K ? t : 0u];
+      const T A = sm.eqA, B = sm.eqB;
+      const T node = sm.nodes[tn];
+      const T coef = sm.ext_coef[t <= (uint32_t)K ? t : 0u];
       // Degree-K extension to node K+1: sum_i (-1)^(K-i) C(K+1, i) t(i).
       const T ext = lanes8_sum<FC>(t <= (uint32_t)K ? FC::mul(tv, coef) : FC::zero());
       if (t == (uint32_t)(K + 1)) tv = ext;
@@ -640,20 +716,17 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   }
   if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
-  const T r = el_from_bytes<FC>(sm.out);
+  const T r = el_from_words<FC>(reinterpret_cast<const uint32_t*>(sm.out));
   T eq_next = FC::zero();
-  if constexpr (EQ) {
-    const T A = a.eq_ab[2 * round], B = a.eq_ab[2 * round + 1];
-    eq_next = FC::mul(sm.eq_prev, FC::add(B, FC::mul(r, A)));
-  }
+  if constexpr (EQ) eq_next = FC::mul(sm.eq_prev, FC::add(sm.eqB, FC::mul(r, sm.eqA)));
   T claim = FC::zero();
   if (need_claim) {
     // s(r) = sum_t s(t) L_t(r) with L_t(r) = lag_inv[t] prod_{j != t} (r - j):
     // lane t forms term t; lanes past the nodes add zero.
-    T basis = ctl->lag_inv[tn];
+    T basis = sm.lag_inv[tn];
 #pragma unroll
     for (int j = 0; j < NPTS; ++j) {
-      const T d = FC::sub(r, ctl->nodes[j]);
+      const T d = FC::sub(r, sm.nodes[j]);
       basis = FC::mul(basis, t == (uint32_t)j ? FC::one() : d);
     }
     const T msg = t < (uint32_t)NPTS ? sm.msg[tn] : FC::zero();
@@ -786,9 +859,13 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
     __syncthreads();
     if (!sh.is_last) return;
     if (threadIdx.x == 0) trace_mark(a.trace, round, 2);
-    if (threadIdx.x < 64) fin_load(sh.sm, a.ctl);
+    if (threadIdx.x < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, round);
+      fin_load(sh.sm, a.ctl);
+    }
     slot_rows_sum<FC, NA>(a.slots, sh);
   } else if (threadIdx.x < 64) {
+    fin_eq<FT, FC, EQ>(sh.sm, a, round);
     fin_load(sh.sm, a.ctl);
   }
   if (threadIdx.x < 64) {
@@ -802,8 +879,10 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
 
 // Multi-block pass of one round of one proof. The streaming loop needs fewer
 // registers than the finisher that one block runs at the end; PASS_WAVES_PER_EU
-// keeps the occupancy of the loop (the finisher may spill a few registers).
-#define PASS_WAVES_PER_EU __attribute__((amdgpu_waves_per_eu(5, 8)))
+// keeps the occupancy of the loop (the finisher may spill a few registers):
+// four waves a SIMD, what a grid of 1024 blocks of 256 threads places on 256
+// compute units.
+#define PASS_WAVES_PER_EU __attribute__((amdgpu_waves_per_eu(4, 8)))
 template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
 __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(StageArgs<FT, FC> a,
                                                             const typename FS::T* src,
@@ -885,6 +964,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_kernel(
     if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl, wr, nxr_log);
     ChalMul<FC> cm;
     cm.init(r_sh);
+    if (blk == 0 && tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, round);
     if (blk == 0 && tid == 0) trace_mark(a.trace, round, 1);
     pass_accumulate<FC, FC, K, EQ, true>(a.work, a.work_stride, a.work, a.work_stride,
                                          1u << (a.n - round), cm, wl, wr, nxr_log,
@@ -1086,6 +1166,7 @@ __device__ __forceinline__ void tail_rounds(const StageArgs<FS, FC>& a, uint32_t
     if (wave_mode && tid >= 64) break;
     ChalMul<FC> cm;
     if (round > 0) cm.init(sh.sm.r);
+    if (tid < 64) fin_eq<FS, FC, EQ>(sh.sm, a, round);
     T acc[NA];
 #pragma unroll
     for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
