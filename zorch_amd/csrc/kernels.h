@@ -302,6 +302,9 @@ struct WeightedSum<BB4, BB> {
   }
 };
 
+#ifndef PF_STEP_BYTES
+#define PF_STEP_BYTES 64
+#endif
 template <class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void pass_accumulate(
     const typename FS::T* src, size_t src_stride, typename FC::T* dst, size_t dst_stride,
@@ -311,7 +314,16 @@ __device__ __forceinline__ void pass_accumulate(
   using V = typename FV::T;
   using T = typename FC::T;
   using S = typename FS::T;
-  constexpr bool WEIGHT_FIRST = EQ && std::is_same<FV, FC>::value;
+  // The first folding round of two base-field tables under a BabyBear^4
+  // challenge r: each node value is a + r b with a, b in the base field
+  // (a = lo + t (hi - lo) over the unfolded entries, b the same over their
+  // differences), so the product of the two factors is c0 + r c1 + r^2 c2
+  // with base-field c0 = a0 a1, c1 = a0 b1 + b0 a1, c2 = b0 b1. Summing the
+  // c_i per node (weighted by eq, lazily) and applying r, r^2 once per thread
+  // replaces five extension products a pair with four base ones.
+  constexpr bool SPLIT = FOLD && K == 2 && std::is_same<FS, BB>::value &&
+                         std::is_same<FC, BB4>::value;
+  constexpr bool WEIGHT_FIRST = EQ && !SPLIT && std::is_same<FV, FC>::value;
   constexpr int NS = FOLD ? 4 : 2;  // source streams per table
   // The 64-bit weighted sums of BabyBear^4 weights on BabyBear values take
   // the registers of two more unrolled pairs.
@@ -324,13 +336,26 @@ __device__ __forceinline__ void pass_accumulate(
   constexpr int PAIR_BYTES = K * NS * (int)sizeof(S);
   constexpr bool PREFETCH = PAIR_BYTES <= 32;
   constexpr int U0 = PassUnroll<FS, K, FOLD>::value;
-  constexpr int U = PREFETCH ? 32 / PAIR_BYTES : WIDE_SUMS && U0 > 2 ? 2 : U0;
+  // (SPLIT's nine lazy sums take the registers of a second prefetched pair.)
+  constexpr int U = PREFETCH ? (SPLIT && EQ ? 1 : PF_STEP_BYTES / PAIR_BYTES)
+                             : WIDE_SUMS && U0 > 2 ? 2 : U0;
   const size_t half = L >> 1;
   const size_t nxr_mask = ((size_t)1 << nxr_log) - 1;
   const bool factored = EQ && wl != nullptr && (stride & nxr_mask) == 0;
   WeightedSum<FC, FV> wsum[K + 1];
 #pragma unroll
   for (int t = 0; t <= K; ++t) wsum[t].init();
+  // SPLIT: csum[t][i] sums coefficient c_i of node t (weighted in eq mode).
+  constexpr int NC = SPLIT ? K + 1 : 1;
+  WeightedSum<FC, FS> csum[SPLIT ? K + 1 : 1][NC];
+  typename FS::T bsum[SPLIT ? K + 1 : 1][NC];
+#pragma unroll
+  for (int t = 0; t < (SPLIT ? K + 1 : 1); ++t)
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      csum[t][i].init();
+      bsum[t][i] = FS::zero();
+    }
   // Eq weight of pair j: wsel[j >> wshift], times wr[j & wmask] when the
   // eq_r factor is not applied per thread (wmask 0 reads a dummy otherwise);
   // all uniform, so the weight loads take no branch.
@@ -360,6 +385,45 @@ __device__ __forceinline__ void pass_accumulate(
   };
   // Fold (FOLD) and evaluate pair j from its sources r with weight w.
   auto pair = [&](size_t j, const S (&r)[K][NS], T w) {
+    if constexpr (SPLIT) {
+      S a[K], da[K], b[K], db[K];  // node-0 values and steps of a and b
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const S s0 = r[q][0], s2 = r[q][1], s1 = r[q][2], s3 = r[q][3];
+        const S e0 = FS::sub(s1, s0), e1 = FS::sub(s3, s2);
+        T* d = dst + (size_t)q * dst_stride;
+        d[j] = FC::add(Lift<FS, FC>::go(s0), FoldMul<FC, FS>::go(cm, e0));
+        d[j + half] = FC::add(Lift<FS, FC>::go(s2), FoldMul<FC, FS>::go(cm, e1));
+        a[q] = s0;
+        da[q] = FS::sub(s2, s0);
+        b[q] = e0;
+        db[q] = FS::sub(e1, e0);
+      }
+#pragma unroll
+      for (int t = 0; t <= K; ++t) {
+        const S c0 = FS::mul(a[0], a[1]);
+        const S c1 = FS::add(FS::mul(a[0], b[1]), FS::mul(b[0], a[1]));
+        const S c2 = FS::mul(b[0], b[1]);
+        if constexpr (EQ) {
+          csum[t][0].add(w, c0);
+          csum[t][1].add(w, c1);
+          csum[t][2].add(w, c2);
+        } else {
+          bsum[t][0] = FS::add(bsum[t][0], c0);
+          bsum[t][1] = FS::add(bsum[t][1], c1);
+          bsum[t][2] = FS::add(bsum[t][2], c2);
+        }
+        if (t < K) {
+#pragma unroll
+          for (int q = 0; q < K; ++q) {
+            a[q] = FS::add(a[q], da[q]);
+            b[q] = FS::add(b[q], db[q]);
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      return;
+    }
     V lo[K], df[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -457,7 +521,18 @@ __device__ __forceinline__ void pass_accumulate(
       }
     }
   }
-  if constexpr (EQ && !WEIGHT_FIRST) {
+  if constexpr (SPLIT) {
+    // node t = C0 + r C1 + r^2 C2.
+    const T r1 = cm.r, r2 = FC::mul(cm.r, cm.r);
+#pragma unroll
+    for (int t = 0; t <= K; ++t) {
+      T c[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+        c[i] = EQ ? csum[t][i].get() : Lift<FS, FC>::go(bsum[t][i]);
+      acc[t] = FC::add(acc[t], FC::add(c[0], FC::add(FC::mul(r1, c[1]), FC::mul(r2, c[2]))));
+    }
+  } else if constexpr (EQ && !WEIGHT_FIRST) {
 #pragma unroll
     for (int t = 0; t <= K; ++t) acc[t] = FC::add(acc[t], wsum[t].get());
   }
