@@ -245,6 +245,28 @@ def bench_serve(args) -> dict:
     def pct(xs, q):
         return float(np.percentile(np.asarray(xs) * 1e3, q)) if xs else None
 
+    # Launch shapes of the served steps: the autotuner times a burst of the
+    # trace's first requests through a server under each candidate (results
+    # never depend on them); the one-proof-per-call runs below use the
+    # context's defaults, tuned for single proofs.
+    single_tune = ctx.get_tune()
+    tuned = None
+    if not args.no_autotune:
+        from zorch_amd import autotune
+        burst = min(400, len(spec.arrivals))
+
+        def served_burst():
+            with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+                for rid in [submit(srv, j) for j in range(burst)]:
+                    srv.wait(rid)
+
+        cands = [dict(max_grid=g, pairs_per_thread=p_, tail_pairs=t)
+                 for g in (256, 512, 1024) for p_ in (1, 4) for t in (1024, 2048, 4096)]
+        best, best_ms, _ = autotune.tune(ctx, served_burst, key="serve_zerocheck",
+                                         candidates=cands, reps=2)
+        tuned = {"best": best, "best_ms": best_ms, "burst": burst, "candidates": len(cands)}
+    served_tune = ctx.get_tune()
+
     # Open loop through the server over the whole trace.
     N = len(spec.arrivals)
     lat, sample = [], {}
@@ -281,6 +303,7 @@ def bench_serve(args) -> dict:
         identical &= _verify_proof(inst, msgs, transcript(k))
 
     # One proof per call, first come first served, over the first part of the trace.
+    ctx.set_tune(**single_tune)
     Np = int(np.searchsorted(spec.arrivals, args.plain_duration))
     jobs = {n: make_job(dev[n], base[n].claim, base[n].chal, eq, base[n].degree, base[n].z)
             for n in base}
@@ -297,19 +320,28 @@ def bench_serve(args) -> dict:
         plain_lat.append(time.perf_counter() - t0 - spec.arrivals[j])
     plain_s = time.perf_counter() - t0
 
-    # Capacity: the first C requests all at once, served versus one per call.
+    # Capacity: the first C requests all at once, served versus one per call;
+    # both prepare their requests (arguments, output buffers, transcripts)
+    # before the clock starts.
     C = min(args.capacity_requests, N)
+    ctx.set_tune(**served_tune)
+    prepared = []
+    for j in range(C):
+        inst = base[spec.sizes[j]]
+        prepared.append((make_job(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, inst.z),
+                         transcript(j)))
     with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
         t0 = time.perf_counter()
-        cids = [submit(srv, j) for j in range(C)]
+        cids = [srv.submit_job(job, tr) for job, tr in prepared]
         for rid in cids:
             srv.wait(rid)
         cap_served = C / (time.perf_counter() - t0)
+    ctx.set_tune(**single_tune)
+    states = [tr.state.copy() for _, tr in prepared]
     t0 = time.perf_counter()
     for j in range(C):
         job = jobs[spec.sizes[j]]
-        st = transcript(j).state.copy()
-        _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
+        _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(states[j]),
                                 ctypes.byref(job.proof)))
     cap_plain = C / (time.perf_counter() - t0)
     elems = sum(2 * (1 << n) for n in spec.sizes)
@@ -337,6 +369,7 @@ def bench_serve(args) -> dict:
         "capacity": {"requests": C, "served_proofs_per_s": cap_served,
                      "plain_proofs_per_s": cap_plain, "speedup": cap_served / cap_plain},
         "identical_to_alone": bool(identical), "sampled": len(sample),
+        "tune": {"served": served_tune, "single": single_tune}, "autotune": tuned,
     }
 
 

@@ -217,15 +217,18 @@ class InstanceImpl final : public Instance {
       const uint32_t L = 1u << (n_ - r);
       const uint32_t need_claim = r + 1 == last ? 1u : 0u;  // the claim is read after `last`
       hipEvent_t ev = begin(s);
+      // Block 0 of a pass kernel finishes the round; grid blocks stream it.
+      const uint32_t blocks = (uint32_t)grid + 1;
       if (r == 0) {
         pass_kernel<FT, FT, FC, K, EQ, false>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
+            <<<blocks, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
       } else if (r == 1) {
         pass_kernel<FT, FT, FC, K, EQ, true>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
+            <<<blocks, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
       } else {
         pass_kernel<FT, FC, FC, K, EQ, true>
-            <<<grid, PASS_THREADS, 0, s>>>(args_, args_.work, args_.work_stride, L, r, need_claim);
+            <<<blocks, PASS_THREADS, 0, s>>>(args_, args_.work, args_.work_stride, L, r,
+                                             need_claim);
       }
       ZS_HIP(hipGetLastError());
       if (ev) mark(s, r, KREC_PASS, (uint32_t)grid, pass_bytes(r), ev);

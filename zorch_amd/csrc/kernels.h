@@ -840,6 +840,14 @@ struct RoundLds {
   uint32_t is_last;
 };
 
+// LDS of a pass kernel: the round's LDS and a scratch copy of the hop state
+// for the finisher block's warm-up run.
+template <class FC, int NA>
+struct PassLds {
+  RoundLds<FC, NA> round;
+  HopLds warm;
+};
+
 // Add a block's node sums (NA elements in LDS), word by word, to slot row
 // blk % NSLOT.
 template <class FC, int NA>
@@ -958,21 +966,10 @@ __device__ __forceinline__ void round_block(const StageArgs<FT, FC>& a, const ty
 // four waves a SIMD, what a grid of 1024 blocks of 256 threads places on 256
 // compute units.
 #define PASS_WAVES_PER_EU __attribute__((amdgpu_waves_per_eu(4, 8)))
-template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
-__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(StageArgs<FT, FC> a,
-                                                            const typename FS::T* src,
-                                                            size_t src_stride, uint32_t L,
-                                                            uint32_t round, uint32_t need_claim) {
-  __shared__ RoundLds<FC, K + 1> sh;
-  round_block<FT, FS, FC, K, EQ, FOLD>(a, src, src_stride, L, round, need_claim != 0, blockIdx.x,
-                                       gridDim.x, sh);
-}
 
-// ------------------------------------------------------------ chained rounds
-// Polls of a chained-kernel wait, each after a short sleep: far above any real
-// wait, the limit only keeps a fault from hanging the device. A wait that
-// reaches it marks the proof faulted (the host reports an error) and the
-// block exits.
+// Polls of a device wait, each after a short sleep: far above any real wait,
+// the limit only keeps a fault from hanging the device. A wait that reaches
+// it marks the proof faulted (the host reports an error) and the block exits.
 constexpr uint32_t CHAIN_SPIN_LIMIT = 1u << 22;
 
 // Wait until *p >= target, then acquire (device scope). False on a fault.
@@ -990,6 +987,87 @@ __device__ __forceinline__ bool chain_wait(uint32_t* p, uint32_t target, uint32_
   __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return false;
 }
+
+// One round as 1 + nblk blocks: blocks 1..nblk stream the round's pairs, add
+// their node sums to the slot rows and arrive at the control block's counter
+// (release); block 0 takes no pairs. While the others stream, block 0 loads
+// the proof state and round constants and runs the hop once on a scratch
+// copy of the transcript state (the same code path as the real hop, whose
+// length depends only on the pending length and the message size), so that
+// the finisher's code sits in its compute unit's instruction cache; then it
+// waits for every arrival, sums and clears the rows and finishes the round.
+// Only block 0 waits, on blocks that never wait, so residency is not needed.
+template <class FT, class FS, class FC, int K, bool EQ, bool FOLD>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_kernel(StageArgs<FT, FC> a,
+                                                            const typename FS::T* src,
+                                                            size_t src_stride, uint32_t L,
+                                                            uint32_t round, uint32_t need_claim) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  __shared__ PassLds<FC, NA> ps;
+  RoundLds<FC, NA>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
+  Ctl<FC>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, round);
+      fin_load(sh.sm, ctl);
+      // Warm-up: the hop on a copy of the stream state; the message is
+      // whatever the (not yet written) message buffer holds.
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    if (tid == 0) trace_mark(a.trace, round, 2);
+    slot_rows_sum<FC, NA>(a.slots, sh);
+    if (tid < 64) {
+      finish_round<FT, FC, K, EQ, HOP_PASS>(a, round, need_claim != 0, sh.out, sh.sm, a.work,
+                                            a.in, round == 0 ? a.in_stride : a.work_stride,
+                                            round == 0);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  T acc[NA];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
+  const T* wl = nullptr;
+  const T* wr = nullptr;
+  uint32_t nxr_log = 0;
+  if constexpr (EQ) select_eq<FC>(round, a.n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl, wr, nxr_log);
+  ChalMul<FC> cm;
+  if constexpr (FOLD) cm.init(ctl->r);
+  if (blk == 0 && tid == 0) trace_mark(a.trace, round, 1);
+  pass_accumulate<FS, FC, K, EQ, FOLD>(src, src_stride, a.work, a.work_stride, L, cm, wl, wr,
+                                       nxr_log, (size_t)blk * PASS_THREADS + tid,
+                                       (size_t)nblk * PASS_THREADS, acc);
+  block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
+  slot_row_add<FC, NA>(a.slots, blk, sh.out);
+  // The slot additions (device-scope atomics) are complete before the
+  // arrival. The folded entries reach the next round through the kernel
+  // boundary; only the final fold of the last round reads them in this
+  // kernel, which takes a releasing arrival (a releasing atomic writes back
+  // the whole L2 of its die, too costly for every block of every round).
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) {
+    if (round + 1 == a.n)
+      __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ------------------------------------------------------------ chained rounds
 
 // Blocks that share round `round` of a chained launch of `grid` blocks: one
 // per PASS_THREADS pairs, at most the grid. Non-increasing in the round, so

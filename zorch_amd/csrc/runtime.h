@@ -64,7 +64,7 @@ constexpr uint32_t TUNE_MAX_GRID = 4096;  // equals the kernels' MAXG
 
 struct TuneParams {
   uint32_t tail_pairs = 1024;   // rounds with at most this many pairs run in the tail kernel
-  uint32_t max_grid = 1024;     // blocks of a multi-block pass (<= MAXG)
+  uint32_t max_grid = 512;      // blocks of a multi-block pass (<= MAXG)
   uint32_t pairs_per_thread = 1;  // target pairs per thread of a multi-block pass
   uint32_t chain_pairs = 1u << 18;  // rounds >= 2 with at most this many pairs (and more than
                                     // tail_pairs) run chained in one persistent launch; 0: off

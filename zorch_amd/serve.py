@@ -39,7 +39,12 @@ class ProofServer:
     def submit(self, tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
                transcript, eq_point: FieldArray | None = None) -> int:
         """Queue one stage; `tables` must stay alive until it completes (kept here)."""
-        job = make_job(tables, claim, chal_field, mode, degree, eq_point)
+        return self.submit_job(make_job(tables, claim, chal_field, mode, degree, eq_point),
+                               transcript)
+
+    def submit_job(self, job, transcript) -> int:
+        """Queue one stage prepared by ``sumcheck._device.make_job`` (its output
+        buffers receive the proof; one job per request)."""
         st = transcript.state.copy()
         rid = ctypes.c_uint64()
         _lib.check(_lib.lib().zs_server_submit(self.handle, ctypes.byref(job.args),
