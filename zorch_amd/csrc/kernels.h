@@ -302,9 +302,8 @@ struct WeightedSum<BB4, BB> {
   }
 };
 
-#ifndef PF_STEP_BYTES
-#define PF_STEP_BYTES 64
-#endif
+// Source bytes a thread loads per step of a double-buffered streaming loop.
+constexpr int PF_STEP_BYTES = 64;
 template <class FS, class FC, int K, bool EQ, bool FOLD>
 __device__ __forceinline__ void pass_accumulate(
     const typename FS::T* src, size_t src_stride, typename FC::T* dst, size_t dst_stride,
