@@ -284,21 +284,29 @@ template <class FC, class FV>
 struct WeightedSum {
   typename FC::T s;
   __device__ __forceinline__ void init() { s = FC::zero(); }
-  __device__ __forceinline__ void add(typename FC::T w, typename FV::T x) {
+  __device__ __forceinline__ void add(typename FC::T w, typename FV::T x, bool = true) {
     s = FC::add(s, MixMul<FC, FV>::go(w, x));
   }
   __device__ __forceinline__ typename FC::T get() const { return s; }
 };
+// Two additions fit between folds: from below p 2^32, adding two products
+// (each < p^2) stays below p (2^32 + 2p) < 2^64, and the fold then brings the
+// high word below p again. `add(w, x, false)` skips the fold; the caller
+// alternates.
 template <>
 struct WeightedSum<BB4, BB> {
   uint64_t s[4];
   __device__ __forceinline__ void init() { s[0] = s[1] = s[2] = s[3] = 0; }
-  __device__ __forceinline__ void add(bb4 w, uint32_t x) {
+  __device__ __forceinline__ void add(bb4 w, uint32_t x, bool fold = true) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s[c] = bb_pre(s[c] + (uint64_t)w.c[c] * x);
+    for (int c = 0; c < 4; ++c) {
+      const uint64_t v = s[c] + (uint64_t)w.c[c] * x;
+      s[c] = fold ? bb_pre(v) : v;
+    }
   }
   __device__ __forceinline__ bb4 get() const {
-    return bb4_make(bb_redc(s[0]), bb_redc(s[1]), bb_redc(s[2]), bb_redc(s[3]));
+    return bb4_make(bb_redc(bb_pre(s[0])), bb_redc(bb_pre(s[1])), bb_redc(bb_pre(s[2])),
+                    bb_redc(bb_pre(s[3])));
   }
 };
 
@@ -383,7 +391,8 @@ __device__ __forceinline__ void pass_accumulate(
     }
   };
   // Fold (FOLD) and evaluate pair j from its sources r with weight w.
-  auto pair = [&](size_t j, const S (&r)[K][NS], T w) {
+  // `fold`: whether the lazy sums fold after this pair (every second pair).
+  auto pair = [&](size_t j, const S (&r)[K][NS], T w, bool fold) {
     if constexpr (SPLIT) {
       S a[K], da[K], b[K], db[K];  // node-0 values and steps of a and b
 #pragma unroll
@@ -404,9 +413,9 @@ __device__ __forceinline__ void pass_accumulate(
         const S c1 = FS::add(FS::mul(a[0], b[1]), FS::mul(b[0], a[1]));
         const S c2 = FS::mul(b[0], b[1]);
         if constexpr (EQ) {
-          csum[t][0].add(w, c0);
-          csum[t][1].add(w, c1);
-          csum[t][2].add(w, c2);
+          csum[t][0].add(w, c0, fold);
+          csum[t][1].add(w, c1, fold);
+          csum[t][2].add(w, c2, fold);
         } else {
           bsum[t][0] = FS::add(bsum[t][0], c0);
           bsum[t][1] = FS::add(bsum[t][1], c1);
@@ -451,7 +460,7 @@ __device__ __forceinline__ void pass_accumulate(
 #pragma unroll
       for (int q = 1; q < K; ++q) prod = FV::mul(prod, lo[q]);
       if constexpr (EQ && !WEIGHT_FIRST) {
-        wsum[t].add(w, prod);
+        wsum[t].add(w, prod, fold);
       } else {
         acc[t] = FC::add(acc[t], Lift<FV, FC>::go(prod));
       }
@@ -493,7 +502,7 @@ __device__ __forceinline__ void pass_accumulate(
     for (int u = 0; u < U; ++u) {
       T w = FC::one();
       if constexpr (EQ) w = wmul ? FC::mul(wa[u], wb[u]) : wa[u];
-      pair(j0 + (size_t)u * stride, raw[u], w);
+      pair(j0 + (size_t)u * stride, raw[u], w, (u & 1) || u == U - 1);
     }
     if constexpr (PREFETCH) {
 #pragma unroll
@@ -516,7 +525,7 @@ __device__ __forceinline__ void pass_accumulate(
           w = wsel[j >> wshift];
           if (wmul) w = FC::mul(w, wr[j & wmask]);
         }
-        pair(j, raw[u], w);
+        pair(j, raw[u], w, true);
       }
     }
   }
