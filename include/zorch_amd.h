@@ -260,7 +260,7 @@ typedef struct zs_server_stats {
   uint64_t submitted, admitted, completed, failed, steps;
   uint64_t queued, active, resident_bytes, peak_resident_bytes, peak_active;
   uint64_t budget_bytes;
-  double busy_s; /* scheduler time inside steps */
+  double busy_s; /* wall time with at least one step queued on the device */
 } zs_server_stats;
 /* hbm_budget_bytes = 0: the context's budget. */
 int zs_server_create(zs_ctx* ctx, uint64_t hbm_budget_bytes, zs_server** out);

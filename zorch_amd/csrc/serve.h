@@ -62,9 +62,15 @@ class ServeKindImpl final : public ServeKind {
 
  public:
   ~ServeKindImpl() override {
-    if (ctx_) {
-      if (stage_dev_) ctx_->pool->release(stage_dev_);
-      if (stage_host_) ctx_->host_pool->release(stage_host_);
+    for (Stage& st : ring_) {
+      if (st.ev) {
+        (void)hipEventSynchronize(st.ev);
+        (void)hipEventDestroy(st.ev);
+      }
+      if (ctx_) {
+        if (st.dev) ctx_->pool->release(st.dev);
+        if (st.host) ctx_->host_pool->release(st.host);
+      }
     }
   }
 
@@ -161,19 +167,25 @@ class ServeKindImpl final : public ServeKind {
                      [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; });
     const size_t args_bytes = align256(m * sizeof(StageArgs<FT, FC>));
     const size_t need = args_bytes + m * sizeof(BatchItem);
-    if (need > stage_cap_) {
-      if (stage_dev_) ctx->pool->release(stage_dev_);
-      if (stage_host_) ctx->host_pool->release(stage_host_);
-      stage_dev_ = stage_host_ = nullptr;
-      stage_cap_ = 0;
+    // The next staging slot of the ring, free once the step that last used
+    // it has run (steps are queued ahead of the device).
+    Stage& st = ring_[ring_next_];
+    ring_next_ = (ring_next_ + 1) % NSTAGE;
+    if (st.ev) ZS_HIP(hipEventSynchronize(st.ev));
+    else ZS_HIP(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+    if (need > st.cap) {
+      if (st.dev) ctx->pool->release(st.dev);
+      if (st.host) ctx->host_pool->release(st.host);
+      st.dev = st.host = nullptr;
+      st.cap = 0;
       size_t cap = 4096;
       while (cap < need) cap *= 2;
-      int rc = ctx->pool->alloc(cap, &stage_dev_);
+      int rc = ctx->pool->alloc(cap, &st.dev);
       if (rc) return rc;
-      if ((rc = ctx->host_pool->alloc(cap, &stage_host_))) return rc;
-      stage_cap_ = cap;
+      if ((rc = ctx->host_pool->alloc(cap, &st.host))) return rc;
+      st.cap = cap;
     }
-    uint8_t* h = (uint8_t*)stage_host_;
+    uint8_t* h = (uint8_t*)st.host;
     StageArgs<FT, FC>* args = (StageArgs<FT, FC>*)h;
     for (size_t i = 0; i < m; ++i) args[i] = static_cast<Req*>(reqs[i])->sa;
     BatchItem* items = (BatchItem*)(h + args_bytes);
@@ -185,11 +197,12 @@ class ServeKindImpl final : public ServeKind {
         items[nitems++] = it;
       }
     }
-    ZS_HIP(hipMemcpyAsync(stage_dev_, h, need, hipMemcpyHostToDevice, s));
-    uint8_t* d = (uint8_t*)stage_dev_;
+    ZS_HIP(hipMemcpyAsync(st.dev, h, need, hipMemcpyHostToDevice, s));
+    uint8_t* d = (uint8_t*)st.dev;
     batch_kernel<FT, FC, K, EQ, 3><<<off, PASS_THREADS, 0, s>>>(
         (const StageArgs<FT, FC>*)d, (const BatchItem*)(d + args_bytes), nitems);
     ZS_HIP(hipGetLastError());
+    ZS_HIP(hipEventRecord(st.ev, s));
     return ZS_OK;
   }
 
@@ -219,10 +232,18 @@ class ServeKindImpl final : public ServeKind {
   }
 
  private:
+  // Staging of a step's argument records and items: one slot per step that
+  // can be queued at once, plus one being filled.
+  struct Stage {
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+  };
+  static constexpr int NSTAGE = 4;
   zs_ctx* ctx_ = nullptr;
-  void* stage_dev_ = nullptr;
-  void* stage_host_ = nullptr;
-  size_t stage_cap_ = 0;
+  Stage ring_[NSTAGE];
+  int ring_next_ = 0;
 };
 
 template <class FT, class FC>

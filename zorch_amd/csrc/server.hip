@@ -4,9 +4,13 @@
 // Each step: admit queued requests in arrival order while their resident
 // state fits the budget, launch the next round of every admitted request
 // (one batched launch per request kind), download the outputs of the requests
-// that complete with this step, wait for the step, and hand those results to
-// pollers. A request therefore joins the batch at the first step after it
-// arrives, whatever the rounds the others are at.
+// that complete with this step and record an event. Up to STEPS_IN_FLIGHT
+// steps are queued on the stream at once: the host builds the next step
+// while the device runs the previous ones (the rounds a step launches are
+// fixed on the host when it is built), and the results of a step are handed
+// to pollers once its event has completed. A request therefore joins the
+// batch at the first step built after it arrives, whatever the rounds the
+// others are at.
 #include <string.h>
 
 #include <chrono>
@@ -53,6 +57,15 @@ struct Active {
   Clock::time_point t_submit;
 };
 
+// A step queued on the stream: its completion event and the requests that
+// complete with it (their outputs are being downloaded).
+struct Flight {
+  hipEvent_t ev = nullptr;
+  std::vector<Active> done;
+  Clock::time_point t_launch;
+};
+constexpr size_t STEPS_IN_FLIGHT = 3;
+
 }  // namespace zs
 
 using namespace zs;
@@ -74,6 +87,9 @@ struct zs_server {
   std::thread th;
   // Scheduler thread only.
   std::vector<Active> active;
+  std::deque<Flight> flights;
+  std::vector<hipEvent_t> free_events;
+  Clock::time_point busy_until{};  // end of the busy time already counted
 };
 
 namespace {
@@ -93,18 +109,54 @@ void publish(zs_server* S, std::vector<std::pair<uint64_t, Finished>>& done, uin
   S->cv_done.notify_all();
 }
 
+// After a device error: wait for the stream, then fail every admitted and
+// in-flight request.
 void fail_all_active(zs_server* S, int rc, const std::string& err) {
+  (void)hipStreamSynchronize(S->ctx->stream);
   std::vector<std::pair<uint64_t, Finished>> done;
   uint64_t freed = 0;
-  for (Active& a : S->active) {
+  auto fail_one = [&](Active& a) {
     a.kind->release(S->ctx, a.req.get());
     freed += a.req->bytes;
     Finished f;
     f.rc = rc;
     f.err = err;
     done.emplace_back(a.id, std::move(f));
-  }
+  };
+  for (Active& a : S->active) fail_one(a);
   S->active.clear();
+  for (Flight& fl : S->flights) {
+    for (Active& a : fl.done) fail_one(a);
+    S->free_events.push_back(fl.ev);
+  }
+  S->flights.clear();
+  publish(S, done, freed);
+}
+
+// Hand the results of the oldest in-flight step to pollers (its event has
+// completed).
+void collect_flight(zs_server* S) {
+  Flight fl = std::move(S->flights.front());
+  S->flights.pop_front();
+  S->free_events.push_back(fl.ev);
+  const auto now = Clock::now();
+  std::vector<std::pair<uint64_t, Finished>> done;
+  uint64_t freed = 0;
+  for (Active& a : fl.done) {
+    Finished f;
+    a.kind->collect(a.req.get(), &f.res);
+    f.latency = std::chrono::duration<double>(now - a.t_submit).count();
+    a.kind->release(S->ctx, a.req.get());
+    freed += a.req->bytes;
+    done.emplace_back(a.id, std::move(f));
+  }
+  // Steps in flight overlap: count the union of [launch, collected] spans.
+  const auto from = fl.t_launch > S->busy_until ? fl.t_launch : S->busy_until;
+  if (now > from) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.busy_s += std::chrono::duration<double>(now - from).count();
+  }
+  if (now > S->busy_until) S->busy_until = now;
   publish(S, done, freed);
 }
 
@@ -116,14 +168,14 @@ void server_loop(zs_server* S) {
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (S->active.empty())
+      if (S->active.empty() && S->flights.empty())
         S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
       if (S->stop) break;
       // Arrival order; a request that can never fit fails instead of blocking.
       while (!S->queue.empty()) {
         Pending& p = S->queue.front();
         if (S->resident + p.bytes > S->budget) {
-          if (!S->active.empty() || !admit.empty()) break;
+          if (!S->active.empty() || !S->flights.empty() || !admit.empty()) break;
           Finished f;
           f.rc = ZS_EBUDGET;
           f.err = "request needs " + std::to_string(p.bytes) + " device bytes; server budget is " +
@@ -143,7 +195,6 @@ void server_loop(zs_server* S) {
       S->stats.resident_bytes = S->resident;
       if (S->resident > S->stats.peak_resident_bytes) S->stats.peak_resident_bytes = S->resident;
     }
-    const auto t_step = Clock::now();
     uint64_t freed = 0;
     for (Pending& p : admit) {
       p.a.claim = p.claim.data();
@@ -172,49 +223,82 @@ void server_loop(zs_server* S) {
     }
     admit.clear();
     if (!done.empty() || freed) publish(S, done, freed);
-    if (S->active.empty()) continue;
-    // One step: the next round of every admitted request, one launch per kind.
-    std::map<ServeKind*, std::vector<ServeReq*>> groups;
-    for (Active& a : S->active) groups[a.kind].push_back(a.req.get());
-    int rc = ZS_OK;
-    for (auto& g : groups)
-      if ((rc = g.first->step(ctx, g.second, ctx->stream))) break;
-    if (!rc) {
-      for (Active& a : S->active)
-        if (a.req->finishing && (rc = a.kind->download(a.req.get(), ctx->stream))) break;
-    }
-    if (!rc) {
-      const hipError_t e = hipStreamSynchronize(ctx->stream);
-      if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
-    }
-    if (rc) {
-      fail_all_active(S, rc, zs_last_error());
-      continue;
-    }
-    const auto now = Clock::now();
-    std::vector<Active> still;
-    for (Active& a : S->active) {
-      if (!a.req->finishing) {
-        still.push_back(std::move(a));
+    if (!S->active.empty() && S->flights.size() < STEPS_IN_FLIGHT) {
+      // One step: the next round of every admitted request, one launch per kind.
+      Flight fl;
+      fl.t_launch = Clock::now();
+      std::map<ServeKind*, std::vector<ServeReq*>> groups;
+      for (Active& a : S->active) groups[a.kind].push_back(a.req.get());
+      int rc = ZS_OK;
+      for (auto& g : groups)
+        if ((rc = g.first->step(ctx, g.second, ctx->stream))) break;
+      if (!rc) {
+        for (Active& a : S->active)
+          if (a.req->finishing && (rc = a.kind->download(a.req.get(), ctx->stream))) break;
+      }
+      if (!rc) {
+        if (S->free_events.empty()) {
+          hipEvent_t e = nullptr;
+          const hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+          if (he != hipSuccess) rc = hip_fail(he, "hipEventCreateWithFlags");
+          else S->free_events.push_back(e);
+        }
+      }
+      if (!rc) {
+        fl.ev = S->free_events.back();
+        S->free_events.pop_back();
+        const hipError_t he = hipEventRecord(fl.ev, ctx->stream);
+        if (he != hipSuccess) {
+          S->free_events.push_back(fl.ev);
+          rc = hip_fail(he, "hipEventRecord");
+        }
+      }
+      if (rc) {
+        fail_all_active(S, rc, zs_last_error());
         continue;
       }
-      Finished f;
-      a.kind->collect(a.req.get(), &f.res);
-      f.latency = std::chrono::duration<double>(now - a.t_submit).count();
-      a.kind->release(ctx, a.req.get());
-      freed += a.req->bytes;
-      done.emplace_back(a.id, std::move(f));
+      std::vector<Active> still;
+      for (Active& a : S->active) {
+        if (a.req->finishing) fl.done.push_back(std::move(a));
+        else still.push_back(std::move(a));
+      }
+      S->active.swap(still);
+      S->flights.push_back(std::move(fl));
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        ++S->stats.steps;
+        S->stats.active = S->active.size();
+      }
+    } else if (!S->flights.empty()) {
+      // Nothing to launch: wait for the oldest step.
+      const hipError_t he = hipEventSynchronize(S->flights.front().ev);
+      if (he != hipSuccess) {
+        fail_all_active(S, hip_fail(he, "hipEventSynchronize"), zs_last_error());
+        continue;
+      }
     }
-    S->active.swap(still);
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      ++S->stats.steps;
-      S->stats.busy_s += std::chrono::duration<double>(now - t_step).count();
+    // Results of every completed step, oldest first.
+    while (!S->flights.empty()) {
+      const hipError_t q = hipEventQuery(S->flights.front().ev);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        fail_all_active(S, hip_fail(q, "hipEventQuery"), zs_last_error());
+        break;
+      }
+      collect_flight(S);
     }
-    publish(S, done, freed);
   }
+  // Stopping: let queued steps finish, then drop every request.
+  (void)hipStreamSynchronize(ctx->stream);
+  for (Flight& fl : S->flights) {
+    for (Active& a : fl.done) a.kind->release(ctx, a.req.get());
+    S->free_events.push_back(fl.ev);
+  }
+  S->flights.clear();
   for (Active& a : S->active) a.kind->release(ctx, a.req.get());
   S->active.clear();
+  for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
+  S->free_events.clear();
 }
 
 int take_result(zs_server* S, std::unique_lock<std::mutex>& lk, uint64_t id, zs_proof* out,
