@@ -262,8 +262,9 @@ def bench_serve(args) -> dict:
                 for rid in [submit(srv, j) for j in range(burst)]:
                     srv.wait(rid)
 
-        cands = [dict(max_grid=g, pairs_per_thread=p_, tail_pairs=t)
-                 for g in (256, 512, 1024) for p_ in (1, 4) for t in (1024, 2048, 4096)]
+        cands = [dict(max_grid=g, pairs_per_thread=p_, tail_pairs=t, step_rounds=s)
+                 for g in (256, 512, 1024) for p_ in (1, 4) for t in (1024, 2048, 4096)
+                 for s in (0, 4)]
         best, best_ms, _ = autotune.tune(ctx, served_burst, key="serve_zerocheck",
                                          candidates=cands, reps=2)
         tuned = {"best": best, "best_ms": best_ms, "burst": burst, "candidates": len(cands)}

@@ -168,6 +168,8 @@ int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
  * and the per-round calls, rounds from the third on with at most chain_pairs
  * pairs (and more than tail_pairs) run in one persistent launch of at most
  * chain_grid blocks (power of two; chain_pairs 0: one launch per round).
+ * In batched and served proving a proof's tail runs at most step_rounds rounds
+ * per step (0: all of its remaining rounds in one step).
  * Results do not depend on them. Set applies to proofs started afterwards. */
 typedef struct zs_tune {
   uint32_t tail_pairs;
@@ -175,7 +177,8 @@ typedef struct zs_tune {
   uint32_t pairs_per_thread;
   uint32_t chain_pairs;
   uint32_t chain_grid;
-  uint32_t reserved[3];
+  uint32_t step_rounds;
+  uint32_t reserved[2];
 } zs_tune;
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out);
 int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* tune);
@@ -252,7 +255,8 @@ int zs_verify_round(uint32_t chal_field, zs_transcript* transcript, const void* 
  * then repeatedly runs one step: the next round of every admitted request,
  * one batched launch per request kind, so that requests which arrive while
  * others are mid-proof join at the next step. A request whose remaining
- * rounds are small finishes them all within one step. Results are those of
+ * rounds are small finishes them within one step (at most step_rounds of
+ * them per step when that launch shape is set). Results are those of
  * zs_prove for the same arguments and transcript. Submit, poll, wait and
  * completed may be called from any thread. */
 typedef struct zs_server zs_server;

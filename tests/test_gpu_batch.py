@@ -83,7 +83,12 @@ def test_mixed_batch_matches_c_oracle():
 
 @pytest.mark.parametrize("tune", [dict(tail_pairs=0, max_grid=4, pairs_per_thread=1),
                                   dict(tail_pairs=64, max_grid=4096, pairs_per_thread=4),
-                                  dict(tail_pairs=1 << 20, max_grid=2048, pairs_per_thread=1)])
+                                  dict(tail_pairs=1 << 20, max_grid=2048, pairs_per_thread=1),
+                                  # tails split over several steps
+                                  dict(tail_pairs=1 << 20, max_grid=512, pairs_per_thread=1,
+                                       step_rounds=1),
+                                  dict(tail_pairs=1024, max_grid=512, pairs_per_thread=2,
+                                       step_rounds=3)])
 def test_batch_independent_of_launch_shapes(tune):
     ctx = default_context()
     insts = _mixed()

@@ -60,10 +60,13 @@ def _requests():
     return insts
 
 
-def test_served_results_equal_alone():
+@pytest.mark.parametrize("step_rounds", [0, 2])
+def test_served_results_equal_alone(step_rounds):
     ctx = Context(0)
     insts = _requests()
     ref = [_alone(i, ctx) for i in insts]
+    # step_rounds 2: a request's tail rounds spread over several steps.
+    ctx.set_tune(step_rounds=step_rounds, tail_pairs=4096)
     with ProofServer(ctx) as srv:
         ids = []
         for k, inst in enumerate(insts):

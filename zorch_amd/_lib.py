@@ -104,7 +104,8 @@ class TuneC(ctypes.Structure):
         ("pairs_per_thread", ctypes.c_uint32),
         ("chain_pairs", ctypes.c_uint32),
         ("chain_grid", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("step_rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 

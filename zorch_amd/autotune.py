@@ -1,9 +1,9 @@
 """Launch-shape autotuner.
 
 The prover's launch shapes (``Context.set_tune``: the largest grid of a
-multi-block pass, the target pairs per thread, and the round size below which
-a proof's remaining rounds run in one block) never change results, only
-speed. ``tune`` times a workload under each candidate shape and keeps the
+multi-block pass, the target pairs per thread, the round size below which
+a proof's remaining rounds run in one block, and in batched and served proving
+how many of those rounds one step runs) never change results, only speed. ``tune`` times a workload under each candidate shape and keeps the
 fastest; results are cached per workload key in a JSON file (``ZS_TUNE_CACHE``
 or ``~/.cache/zorch_amd/tune.json``) and re-applied with ``apply``.
 """
@@ -22,9 +22,9 @@ from zorch_amd.runtime import Context
 
 ARCH = "gfx950"
 
-CANDIDATES = [dict(max_grid=g, pairs_per_thread=p, tail_pairs=t)
-              for g, p, t in itertools.product((256, 512, 1024, 2048, 4096), (1, 2, 4),
-                                               (512, 1024, 2048, 4096))]
+CANDIDATES = [dict(max_grid=g, pairs_per_thread=p, tail_pairs=t, step_rounds=s)
+              for g, p, t, s in itertools.product((256, 512, 1024, 2048, 4096), (1, 2, 4),
+                                                  (512, 1024, 2048, 4096), (0, 4))]
 
 
 def cache_path() -> Path:
@@ -62,7 +62,8 @@ def apply(ctx: Context, key: str) -> dict | None:
     got = load(key)
     if got is None:
         return None
-    params = {k: got[k] for k in ("max_grid", "pairs_per_thread", "tail_pairs")}
+    params = {k: got[k] for k in ("max_grid", "pairs_per_thread", "tail_pairs", "step_rounds")
+              if k in got}
     ctx.set_tune(**params)
     return params
 

@@ -241,22 +241,26 @@ class BatchGroupImpl final : public BatchGroup {
   }
 
  private:
-  // Items of every step: round s of each running member; tails first (they
-  // run longest), then passes by decreasing share of the grid.
+  // Items of every step: the next round of each running member, or the next
+  // run of at most step_rounds tail rounds (all of them when 0); tails first
+  // (they run longest), then passes by decreasing share of the grid. A pass
+  // item of step s is round s of its member.
   void build_schedule() {
-    std::vector<char> done(members_.size(), 0);
+    std::vector<uint32_t> next(members_.size(), 0);
     const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
     for (uint32_t st = 0; st < max_n_; ++st) {
       std::vector<BatchItem> tails, passes;
       Step S;
       for (size_t i = 0; i < members_.size(); ++i) {
-        const uint32_t n = members_[i].n;
-        if (done[i] || n <= st) continue;
-        const size_t pairs = (size_t)1 << (n - st - 1);
+        const uint32_t n = members_[i].n, round = next[i];
+        if (round >= n) continue;
+        const size_t pairs = (size_t)1 << (n - round - 1);
         if (pairs <= tune_.tail_pairs) {
-          tails.push_back(BatchItem{(uint32_t)i, 1, 0, st | TAIL_FLAG});
-          done[i] = 1;
-          for (uint32_t r = st; r < n; ++r) S.bytes += pass_bytes(n, r);
+          const uint32_t end =
+              tune_.step_rounds && n - round > tune_.step_rounds ? round + tune_.step_rounds : n;
+          tails.push_back(BatchItem{(uint32_t)i, 1, 0, tail_item(round, end)});
+          next[i] = end;
+          for (uint32_t r = round; r < end; ++r) S.bytes += pass_bytes(n, r);
           continue;
         }
         const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
@@ -265,6 +269,7 @@ class BatchGroupImpl final : public BatchGroup {
         if (grid < 1) grid = 1;
         while (grid & (grid - 1)) grid &= grid - 1;
         passes.push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, st});
+        next[i] = st + 1;
         S.bytes += pass_bytes(n, st);
       }
       std::stable_sort(passes.begin(), passes.end(),

@@ -339,7 +339,7 @@ int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out) {
   if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
   *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread,
-                 ctx->tune.chain_pairs, ctx->tune.chain_grid, {0, 0, 0}};
+                 ctx->tune.chain_pairs, ctx->tune.chain_grid, ctx->tune.step_rounds, {0, 0}};
   return ZS_OK;
 }
 
@@ -358,6 +358,7 @@ int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* t) {
   ctx->tune.pairs_per_thread = t->pairs_per_thread;
   ctx->tune.chain_pairs = t->chain_pairs;
   ctx->tune.chain_grid = t->chain_grid;
+  ctx->tune.step_rounds = t->step_rounds;
   return ZS_OK;
 }
 

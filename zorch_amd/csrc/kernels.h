@@ -1431,15 +1431,20 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
 // ------------------------------------------------------------ batched steps
 // One step over many proofs: each item is one round of one proof. Item i owns
 // blocks [block_off, block_off + blocks) of the launch; a tail item is one
-// block that runs all of its proof's remaining rounds. Items are sorted by
+// block that runs a run of its proof's small rounds. Items are sorted by
 // block_off.
 struct BatchItem {
   uint32_t inst;        // index into the step's StageArgs array
   uint32_t blocks;      // power of two
   uint32_t block_off;
-  uint32_t round_tail;  // round | TAIL_FLAG: rounds [round, n) in this block
+  uint32_t round_tail;  // a pass: round; a tail: tail_item(round, end), rounds [round, end)
 };
 constexpr uint32_t TAIL_FLAG = 0x80000000u;
+constexpr uint32_t ROUND_MASK = 0xffffu;
+constexpr uint32_t TAIL_END_SHIFT = 16;
+ZS_HD constexpr uint32_t tail_item(uint32_t round, uint32_t end) {
+  return round | end << TAIL_END_SHIFT | TAIL_FLAG;
+}
 
 // Item of this block: the number of items whose block_off <= blockIdx.x,
 // minus one, counted by the whole block in chunks of PASS_THREADS items.
@@ -1458,15 +1463,15 @@ union BatchLds {
   TailLds<FC, NA, PASS_THREADS> tail;
 };
 
-// A tail item of a batch: all remaining rounds of one proof in this block.
+// A tail item of a batch: rounds [round, end) of one proof in this block.
 // Out of line, so that the tail's registers do not weigh on the streaming
 // loop of the pass items.
 template <class FT, class FC, int K, bool EQ>
 static __device__ __attribute__((noinline)) void batch_tail(
-    const StageArgs<FT, FC>* __restrict__ args, uint32_t round,
+    const StageArgs<FT, FC>* __restrict__ args, uint32_t round, uint32_t end,
     ZS_LDS TailLds<FC, K + 1, PASS_THREADS>* sh) {
   const StageArgs<FT, FC> a = *args;
-  tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, a.n,
+  tail_rounds<FT, FC, K, EQ, PASS_THREADS>(a, round, end,
                                            *(TailLds<FC, K + 1, PASS_THREADS>*)sh);
 }
 
@@ -1482,9 +1487,10 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
   const uint32_t idx = find_item(items, nitems);
   const BatchItem it = items[idx];
   const StageArgs<FT, FC> a = args[it.inst];
-  const uint32_t round = it.round_tail & ~TAIL_FLAG;
+  const uint32_t round = it.round_tail & ROUND_MASK;
   if (it.round_tail & TAIL_FLAG) {
     batch_tail<FT, FC, K, EQ>(args + it.inst, round,
+                              (it.round_tail & ~TAIL_FLAG) >> TAIL_END_SHIFT,
                               (ZS_LDS TailLds<FC, K + 1, PASS_THREADS>*)&sh.tail);
     return;
   }

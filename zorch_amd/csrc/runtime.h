@@ -69,6 +69,7 @@ struct TuneParams {
   uint32_t chain_pairs = 1u << 18;  // rounds >= 2 with at most this many pairs (and more than
                                     // tail_pairs) run chained in one persistent launch; 0: off
   uint32_t chain_grid = 128;        // blocks of a chained launch (at most)
+  uint32_t step_rounds = 0;         // batched / served: tail rounds per step (0: all)
 };
 
 // Kernel timing records (enabled by zs_ctx_profile).

@@ -150,8 +150,12 @@ class ServeKindImpl final : public ServeKind {
       const uint32_t round = r->next;
       const size_t pairs = (size_t)1 << (r->n - round - 1);
       if (pairs <= tune.tail_pairs) {
-        tails.push_back(BatchItem{(uint32_t)i, 1, 0, round | TAIL_FLAG});
-        r->next = r->n;
+        // A run of at most step_rounds tail rounds (all of them when 0).
+        const uint32_t end = tune.step_rounds && r->n - round > tune.step_rounds
+                                 ? round + tune.step_rounds
+                                 : r->n;
+        tails.push_back(BatchItem{(uint32_t)i, 1, 0, tail_item(round, end)});
+        r->next = end;
       } else {
         const size_t per_block = (size_t)PASS_THREADS * tune.pairs_per_thread;
         size_t grid = (pairs + per_block - 1) / per_block;

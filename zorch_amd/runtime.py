@@ -41,12 +41,12 @@ class Context:
 
     def get_tune(self) -> dict:
         """Launch shapes of later proofs: tail_pairs, max_grid, pairs_per_thread,
-        chain_pairs, chain_grid."""
+        chain_pairs, chain_grid, step_rounds."""
         t = _lib.TuneC()
         _lib.check(_lib.lib().zs_ctx_get_tune(self.handle, ctypes.byref(t)))
         return {"tail_pairs": t.tail_pairs, "max_grid": t.max_grid,
                 "pairs_per_thread": t.pairs_per_thread, "chain_pairs": t.chain_pairs,
-                "chain_grid": t.chain_grid}
+                "chain_grid": t.chain_grid, "step_rounds": t.step_rounds}
 
     def set_tune(self, **kw) -> None:
         """Change launch shapes (results never depend on them)."""
@@ -56,7 +56,7 @@ class Context:
             raise ValueError(f"unknown tuning keys {sorted(unknown)}")
         cur.update(kw)
         t = _lib.TuneC(cur["tail_pairs"], cur["max_grid"], cur["pairs_per_thread"],
-                       cur["chain_pairs"], cur["chain_grid"])
+                       cur["chain_pairs"], cur["chain_grid"], cur["step_rounds"])
         _lib.check(_lib.lib().zs_ctx_set_tune(self.handle, ctypes.byref(t)))
 
     def close(self) -> None:
