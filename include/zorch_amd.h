@@ -169,7 +169,10 @@ int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
  * pairs (and more than tail_pairs) run in one persistent launch of at most
  * chain_grid blocks (power of two; chain_pairs 0: one launch per round).
  * In batched and served proving a proof's tail runs at most step_rounds rounds
- * per step (0: all of its remaining rounds in one step).
+ * per step (0: all of its remaining rounds in one step). lookahead (0/1): in
+ * zs_prove, the single-block tail (when it starts at the third round or later
+ * and runs at least two rounds) folds and evaluates each round while the
+ * previous round's message is hashed.
  * Results do not depend on them. Set applies to proofs started afterwards. */
 typedef struct zs_tune {
   uint32_t tail_pairs;
@@ -178,7 +181,8 @@ typedef struct zs_tune {
   uint32_t chain_pairs;
   uint32_t chain_grid;
   uint32_t step_rounds;
-  uint32_t reserved[2];
+  uint32_t lookahead;
+  uint32_t reserved[1];
 } zs_tune;
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out);
 int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* tune);

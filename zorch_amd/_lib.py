@@ -105,7 +105,8 @@ class TuneC(ctypes.Structure):
         ("chain_pairs", ctypes.c_uint32),
         ("chain_grid", ctypes.c_uint32),
         ("step_rounds", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("lookahead", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 

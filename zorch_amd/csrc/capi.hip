@@ -339,7 +339,8 @@ int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int zs_ctx_get_tune(zs_ctx* ctx, zs_tune* out) {
   if (!ctx || !out) return fail(ZS_EINVAL, "null argument");
   *out = zs_tune{ctx->tune.tail_pairs, ctx->tune.max_grid, ctx->tune.pairs_per_thread,
-                 ctx->tune.chain_pairs, ctx->tune.chain_grid, ctx->tune.step_rounds, {0, 0}};
+                 ctx->tune.chain_pairs, ctx->tune.chain_grid, ctx->tune.step_rounds,
+                 ctx->tune.lookahead, {0}};
   return ZS_OK;
 }
 
@@ -353,12 +354,14 @@ int zs_ctx_set_tune(zs_ctx* ctx, const zs_tune* t) {
   const uint32_t cg = t->chain_grid;
   if (cg == 0 || cg > zs::TUNE_MAX_GRID || (cg & (cg - 1)))
     return fail(ZS_EINVAL, "chain_grid must be a power of two in [1, 4096]");
+  if (t->lookahead > 1) return fail(ZS_EINVAL, "lookahead must be 0 or 1");
   ctx->tune.tail_pairs = t->tail_pairs;
   ctx->tune.max_grid = g;
   ctx->tune.pairs_per_thread = t->pairs_per_thread;
   ctx->tune.chain_pairs = t->chain_pairs;
   ctx->tune.chain_grid = t->chain_grid;
   ctx->tune.step_rounds = t->step_rounds;
+  ctx->tune.lookahead = t->lookahead;
   return ZS_OK;
 }
 

@@ -189,7 +189,10 @@ class InstanceImpl final : public Instance {
       const size_t pairs = (size_t)1 << (n_ - r - 1);
       if (pairs <= tune_.tail_pairs) {
         hipEvent_t ev = begin(s);
-        tail_kernel<FT, FC, K, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
+        if (tune_.lookahead && r >= 2 && last == n_ && n_ - r >= 2)
+          tail_la_kernel<FT, FC, K, EQ><<<1, LA_THREADS, 0, s>>>(args_, r);
+        else
+          tail_kernel<FT, FC, K, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
         ZS_HIP(hipGetLastError());
         if (ev) {
           uint64_t b = 0;

@@ -1428,6 +1428,377 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
   tail_rounds<FS, FC, K, EQ, TAIL_THREADS>(a, first, last, sh);
 }
 
+// ---------------------------------------------------- look-ahead tail rounds
+// The tail rounds [first, n) (first >= 2, at least two rounds) of one proof
+// in one block, with each round's folding and evaluation run while the
+// finisher wave hashes the previous round's message.
+//
+// Round k+1's tables are round k's folded at the challenge r_k, so each of
+// their entries is a + r_k b with a, b entries of round k's tables, and the
+// node sums of round k+1 are polynomials of degree K in r_k. Period P_k folds
+// round k-1's tables at r_{k-1} into round k's (in place in `work`) and sums,
+// per node, the coefficients C_{k+1} of that polynomial over round k+1's
+// pairs; it needs only r_{k-1}, so it runs while wave 0 turns round k's
+// message (from C_k and r_{k-1}) into r_k. The first period, which starts with
+// the kernel, runs in two parts: every wave folds and sums round `first`'s
+// node values directly, then, while the finisher hashes that message, the
+// worker waves sum C_{first+1} from the folded tables. The last period
+// (k = n-1) only folds, for the final fold. Thread g of a period takes round
+// k+1's pair g: eight entries per table of round k-1's tables, at g + m L/8
+// (L their length, m = 0..7), give the four entries of round k's at
+// g + m L/8 (m = 0..3), which hold round k's pairs g and g + L/8 and, folded
+// at r_k, round k+1's pair g.
+//
+// Wave 0 is the finisher. The first period runs on every wave; later ones on
+// waves 1-3 and 5-7, off the SIMD of wave 0 (waves are dealt to the four SIMDs
+// in turn), whose hash is bound by its own instruction issue. Each wave of a
+// period reduces its sums into an LDS row of the period's parity and arrives
+// on an LDS counter; the finisher waits for every arrival of period k before
+// publishing r_k and the start of period k+1 (period k+1 reads round k's
+// tables, complete only then). Every wait is bounded; one that reaches its
+// limit marks the proof faulted.
+constexpr int LA_THREADS = 512;
+constexpr uint32_t LA_FIRST_WAVES = LA_THREADS / 64;  // waves of the first period
+constexpr uint32_t LA_WAVES = 6;                      // waves of the later periods
+constexpr uint32_t LA_SPIN_LIMIT = 1u << 20;
+
+template <class FC, int K>
+struct TailLaLds {
+  static constexpr int NA = K + 1;
+  static constexpr int NP = NA * (K + 2);  // per wave: round sums (NA), then coefficients
+  FinishLds<FC> sm;
+  typename FC::T part[2][LA_FIRST_WAVES][NP];
+  typename FC::T coef[NA * NA];
+  typename FC::T out[NA];
+  typename FC::T pub_r;  // r_{k-1} for period k
+  uint32_t go;           // periods allowed to start: every k < go
+  uint32_t arrived;      // wave arrivals over all periods so far
+  uint32_t fault;
+};
+
+// Spin on an LDS word until it reaches `target` (relaxed polls, then an
+// acquire fence for the work-group). False when the limit is reached.
+__device__ __forceinline__ bool la_wait(uint32_t* p, uint32_t target) {
+  for (uint32_t i = 0; i < LA_SPIN_LIMIT; ++i) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      return true;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return false;
+}
+
+// Eq weight of round `round`'s pair j.
+template <class FC>
+__device__ __forceinline__ typename FC::T la_weight(const typename FC::T* wl,
+                                                    const typename FC::T* wr, uint32_t nxr_log,
+                                                    uint32_t j) {
+  if (wl) return FC::mul(wl[j >> nxr_log], wr[j & ((1u << nxr_log) - 1u)]);
+  return wr[j];
+}
+
+// Coefficients of prod_q (A_q + r B_q) in r, degree K.
+template <class FC, int K>
+__device__ __forceinline__ void la_poly(const typename FC::T (&A)[K], const typename FC::T (&B)[K],
+                                        typename FC::T (&c)[K + 1]) {
+  using T = typename FC::T;
+  if constexpr (K == 2) {
+    c[0] = FC::mul(A[0], A[1]);
+    c[2] = FC::mul(B[0], B[1]);
+    const T m = FC::mul(FC::add(A[0], B[0]), FC::add(A[1], B[1]));
+    c[1] = FC::sub(FC::sub(m, c[0]), c[2]);
+  } else {
+    c[0] = A[0];
+    c[1] = B[0];
+#pragma unroll
+    for (int i = 2; i <= K; ++i) c[i] = FC::zero();
+#pragma unroll
+    for (int q = 1; q < K; ++q) {
+      // (c0 + c1 r + ... + c_q r^q)(A_q + r B_q)
+#pragma unroll
+      for (int i = q + 1; i >= 1; --i) c[i] = FC::add(FC::mul(c[i], A[q]), FC::mul(c[i - 1], B[q]));
+      c[0] = FC::mul(c[0], A[q]);
+    }
+  }
+}
+
+// Parts of a period: fold round k-1's tables at cm.r into round k's and sum
+// round k's node values into d (LA_FOLD_D); sum the coefficients of round
+// k+1's node sums into c[t * (K+1) + i] from round k's tables (LA_C); both
+// fold and coefficients (LA_FOLD_C; the fold alone when k = n-1).
+constexpr int LA_FOLD_D = 0, LA_C = 1, LA_FOLD_C = 2;
+
+// Period P_k (part MODE) for thread wt of nt.
+template <class FS, class FC, int K, bool EQ, int MODE>
+__device__ __forceinline__ void la_period(const StageArgs<FS, FC>& a, uint32_t k, uint32_t wt,
+                                          uint32_t nt, const ChalMul<FC>& cm,
+                                          typename FC::T (&d)[K + 1],
+                                          typename FC::T (&c)[(K + 1) * (K + 1)]) {
+  using T = typename FC::T;
+  const uint32_t n = a.n;
+  if (MODE == LA_FOLD_C && k + 1 >= n) {
+    // Round n-1's two entries per table from round n-2's four.
+    if (wt < 2u * K) {
+      T* s = a.work + (size_t)(wt >> 1) * a.work_stride;
+      const uint32_t y = wt & 1u;
+      const T lo = s[y], hi = s[y + 2];
+      s[y] = FC::add(lo, cm.mul(FC::sub(hi, lo)));
+    }
+    return;
+  }
+  const uint32_t q8 = 1u << (n - k - 2);  // L/8 = round k+1's pairs
+  const T* wl_d = nullptr;
+  const T* wr_d = nullptr;
+  const T* wl_c = nullptr;
+  const T* wr_c = nullptr;
+  uint32_t xd = 0, xc = 0;
+  if constexpr (EQ) {
+    select_eq<FC>(k, n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl_d, wr_d, xd);
+    select_eq<FC>(k + 1, n, a.eq_l_fam, a.eq_r_fam, &a.ctl->one, wl_c, wr_c, xc);
+  }
+  for (uint32_t g = wt; g < q8; g += nt) {
+    T e[K][4];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      T* s = a.work + (size_t)q * a.work_stride;
+      if constexpr (MODE == LA_C) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) e[q][m] = s[g + (size_t)m * q8];
+      } else {
+        T x[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) x[m] = s[g + (size_t)m * q8];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          e[q][m] = FC::add(x[m], cm.mul(FC::sub(x[m + 4], x[m])));
+          s[g + (size_t)m * q8] = e[q][m];
+        }
+      }
+    }
+    if constexpr (MODE == LA_FOLD_D) {
+      // Round k's pairs g = (e0, e2) and g + L/8 = (e1, e3).
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        T lo[K], df[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          lo[q] = e[q][h];
+          df[q] = FC::sub(e[q][h + 2], e[q][h]);
+        }
+        if constexpr (EQ) {
+          const T w = la_weight<FC>(wl_d, wr_d, xd, g + (uint32_t)h * q8);
+          const T hi0 = FC::add(lo[0], df[0]);
+          lo[0] = FC::mul(w, lo[0]);
+          df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
+        }
+#pragma unroll
+        for (int t = 0; t <= K; ++t) {
+          T prod = lo[0];
+#pragma unroll
+          for (int q = 1; q < K; ++q) prod = FC::mul(prod, lo[q]);
+          d[t] = FC::add(d[t], prod);
+          if (t < K) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) lo[q] = FC::add(lo[q], df[q]);
+          }
+        }
+      }
+      continue;
+    }
+    // Round k+1's pair g: lo = a0 + r b0, hi = a1 + r b1 per table.
+    T A[K], B[K], dA[K], dB[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      A[q] = e[q][0];
+      B[q] = FC::sub(e[q][2], e[q][0]);
+      dA[q] = FC::sub(e[q][1], e[q][0]);
+      dB[q] = FC::sub(FC::sub(e[q][3], e[q][1]), B[q]);
+    }
+    if constexpr (EQ) {
+      const T w = la_weight<FC>(wl_c, wr_c, xc, g);
+      A[0] = FC::mul(w, A[0]);
+      B[0] = FC::mul(w, B[0]);
+      dA[0] = FC::mul(w, dA[0]);
+      dB[0] = FC::mul(w, dB[0]);
+    }
+#pragma unroll
+    for (int t = 0; t <= K; ++t) {
+      T p[K + 1];
+      la_poly<FC, K>(A, B, p);
+#pragma unroll
+      for (int i = 0; i <= K; ++i) c[t * (K + 1) + i] = FC::add(c[t * (K + 1) + i], p[i]);
+      if (t < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          A[q] = FC::add(A[q], dA[q]);
+          B[q] = FC::add(B[q], dB[q]);
+        }
+      }
+    }
+  }
+}
+
+// Sum a period's values over the wave into row `row` (lane 0 writes) and
+// arrive. A wave with no pairs (wave_base >= pairs) writes zeros.
+template <class FC, int NV>
+__device__ __forceinline__ void la_arrive(typename FC::T (&v)[NV], bool any,
+                                          typename FC::T* row, uint32_t* arrived) {
+  const uint32_t lane = threadIdx.x & 63;
+  if (any) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      typename FC::T x = v[i];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) x = FC::add(x, shfl_xor_el<FC>(x, m));
+      v[i] = x;
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) row[i] = any ? v[i] : FC::zero();
+  }
+  // The row and this wave's table entries are written before the arrival.
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class FS, class FC, int K, bool EQ>
+__global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a,
+                                                             uint32_t first) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1, NC = NA * NA;
+  __shared__ TailLaLds<FC, K> sh;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t n = a.n;
+  if (tid < 64) fin_load(sh.sm, a.ctl);
+  if (tid == 0) {
+    sh.go = first + 1;
+    sh.arrived = 0;
+    sh.fault = 0;
+    sh.pub_r = a.ctl->r;
+  }
+  __syncthreads();
+  // Wave index among the later periods' waves (1-3, 5-7 -> 0..5), or -1.
+  const int widx = wave == 0 || wave == 4 ? -1 : (int)(wave < 4 ? wave - 1 : wave - 2);
+  const uint32_t q8f = 1u << (n - first - 2);  // pairs of round first+1
+  // Every wave: the first period's fold and round first's node sums.
+  {
+    ChalMul<FC> cm;
+    cm.init(sh.pub_r);
+    T d[NA], c[NC];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) d[t] = FC::zero();
+    la_period<FS, FC, K, EQ, LA_FOLD_D>(a, first, tid, LA_THREADS, cm, d, c);
+    la_arrive<FC, NA>(d, wave * 64u < q8f, sh.part[first & 1][wave], &sh.arrived);
+  }
+  if (widx >= 0) {
+    // Round first+1's coefficients, once every wave has folded.
+    if (!la_wait(&sh.arrived, LA_FIRST_WAVES)) {
+      sh.fault = 1;
+      return;
+    }
+    {
+      ChalMul<FC> cm;
+      T d[NA], c[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) c[i] = FC::zero();
+      const uint32_t wt = (uint32_t)widx * 64u + lane;
+      la_period<FS, FC, K, EQ, LA_C>(a, first, wt, LA_WAVES * 64u, cm, d, c);
+      la_arrive<FC, NC>(c, (uint32_t)widx * 64u < q8f, sh.part[first & 1][widx] + NA,
+                        &sh.arrived);
+    }
+    // Later periods.
+    for (uint32_t k = first + 1; k < n; ++k) {
+      if (!la_wait(&sh.go, k + 1)) {
+        sh.fault = 1;
+        return;
+      }
+      if (__hip_atomic_load(&sh.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return;
+      ChalMul<FC> cm;
+      cm.init(sh.pub_r);
+      T d[NA], c[NC];
+#pragma unroll
+      for (int t = 0; t < NA; ++t) d[t] = FC::zero();
+#pragma unroll
+      for (int i = 0; i < NC; ++i) c[i] = FC::zero();
+      const uint32_t wt = (uint32_t)widx * 64u + lane;
+      la_period<FS, FC, K, EQ, LA_FOLD_C>(a, k, wt, LA_WAVES * 64u, cm, d, c);
+      const uint32_t q8 = k + 2 <= n ? 1u << (n - k - 2) : 0u;
+      la_arrive<FC, NC>(c, k + 1 < n && (uint32_t)widx * 64u < q8,
+                        sh.part[k & 1][widx] + NA, &sh.arrived);
+    }
+    return;
+  }
+  if (wave != 0) return;
+  // The finisher.
+  uint32_t target = LA_FIRST_WAVES;
+  bool ok = true;
+  for (uint32_t k = first; k < n && ok; ++k) {
+    if (lane == 0) trace_mark(a.trace, k, 1);
+    fin_eq<FS, FC, EQ>(sh.sm, a, k);
+    if (k == first) {
+      ok = la_wait(&sh.arrived, target);
+      if (!ok) break;
+      // Round first's node sums, from every wave of the first period.
+      if (lane < (uint32_t)NA) {
+        T s = FC::zero();
+        for (uint32_t w = 0; w < LA_FIRST_WAVES; ++w) s = FC::add(s, sh.part[first & 1][w][lane]);
+        sh.out[lane] = s;
+      }
+    } else {
+      // Round k's node sums: the coefficients of period k-1 at r_{k-1}.
+      const uint32_t par = (k - 1) & 1u;
+      if (lane < (uint32_t)NC) {
+        T s = FC::zero();
+        for (uint32_t w = 0; w < LA_WAVES; ++w) s = FC::add(s, sh.part[par][w][NA + lane]);
+        sh.coef[lane] = s;
+      }
+      wave_sync();
+      if (lane < (uint32_t)NA) {
+        const T r = sh.sm.r;
+        T v = sh.coef[lane * NA + K];
+#pragma unroll
+        for (int i = K - 1; i >= 0; --i) v = FC::add(sh.coef[lane * NA + i], FC::mul(r, v));
+        sh.out[lane] = v;
+      }
+    }
+    wave_sync();
+    StageArgs<FS, FC> af = a;
+    af.final_out = nullptr;  // the final fold waits for the last period
+    finish_round<FS, FC, K, EQ, HOP_INLINE>(af, k, k + 1 == n, sh.out, sh.sm, a.work, a.in,
+                                            a.work_stride, false);
+    // Period k is complete: publish r_k and start period k+1.
+    target += LA_WAVES;
+    ok = la_wait(&sh.arrived, target);
+    if (!ok) break;
+    if (k + 1 < n) {
+      if (lane == 0) {
+        sh.pub_r = sh.sm.r;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&sh.go, k + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      wave_sync();
+    }
+  }
+  if (!ok) {
+    if (lane == 0) {
+      __hip_atomic_store(&sh.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(&a.ctl->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  // The final fold of round n-1's entries (written by the last period).
+  if (a.final_out && lane < (uint32_t)K) {
+    const T* s = a.work + (size_t)lane * a.work_stride;
+    const T lo = s[0], hi = s[1];
+    a.final_out[lane] = FC::add(lo, FC::mul(sh.sm.r, FC::sub(hi, lo)));
+  }
+  if (EQ && a.final_out && lane == 0) a.final_out[K] = sh.sm.eq_prev;
+  fin_store(sh.sm, a.ctl);
+}
+
 // ------------------------------------------------------------ batched steps
 // One step over many proofs: each item is one round of one proof. Item i owns
 // blocks [block_off, block_off + blocks) of the launch; a tail item is one

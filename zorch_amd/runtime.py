@@ -41,12 +41,13 @@ class Context:
 
     def get_tune(self) -> dict:
         """Launch shapes of later proofs: tail_pairs, max_grid, pairs_per_thread,
-        chain_pairs, chain_grid, step_rounds."""
+        chain_pairs, chain_grid, step_rounds, lookahead."""
         t = _lib.TuneC()
         _lib.check(_lib.lib().zs_ctx_get_tune(self.handle, ctypes.byref(t)))
         return {"tail_pairs": t.tail_pairs, "max_grid": t.max_grid,
                 "pairs_per_thread": t.pairs_per_thread, "chain_pairs": t.chain_pairs,
-                "chain_grid": t.chain_grid, "step_rounds": t.step_rounds}
+                "chain_grid": t.chain_grid, "step_rounds": t.step_rounds,
+                "lookahead": t.lookahead}
 
     def set_tune(self, **kw) -> None:
         """Change launch shapes (results never depend on them)."""
@@ -56,7 +57,8 @@ class Context:
             raise ValueError(f"unknown tuning keys {sorted(unknown)}")
         cur.update(kw)
         t = _lib.TuneC(cur["tail_pairs"], cur["max_grid"], cur["pairs_per_thread"],
-                       cur["chain_pairs"], cur["chain_grid"], cur["step_rounds"])
+                       cur["chain_pairs"], cur["chain_grid"], cur["step_rounds"],
+                       cur["lookahead"])
         _lib.check(_lib.lib().zs_ctx_set_tune(self.handle, ctypes.byref(t)))
 
     def close(self) -> None:
