@@ -189,20 +189,27 @@ def test_chained_rounds_equal_one_launch_per_round(tune):
     _compare_with_c(insts[0])
 
 
-@pytest.mark.parametrize("tail_pairs", [2, 64, 1024, 4096])
-def test_lookahead_tail_equals_plain_tail(tail_pairs):
-    """The tail that folds each round while the previous message is hashed
-    equals the round-by-round tail, bit for bit, over fields, factor counts,
-    both modes, both transcripts and tails that start from the third round to
-    two rounds before the end; the C oracle anchors one of them."""
+@pytest.mark.parametrize("tune", [
+    dict(tail_pairs=2), dict(tail_pairs=64), dict(tail_pairs=1024), dict(tail_pairs=4096),
+    # chained rounds too: to the tail, to the last round, over few blocks
+    dict(tail_pairs=64, chain_pairs=1 << 20),
+    dict(tail_pairs=0, chain_pairs=1 << 20),
+    dict(tail_pairs=8, chain_pairs=1 << 20, max_grid=4),
+    dict(tail_pairs=256, chain_pairs=1 << 12, pairs_per_thread=4),
+])
+def test_lookahead_equals_plain_rounds(tune):
+    """Tails and chained launches that fold each round while the previous
+    message is hashed equal the round-by-round kernels, bit for bit, over
+    fields, factor counts, both modes, both transcripts and every start and
+    end round; the C oracle anchors one of them."""
     insts = [W.zerocheck_bb4(15, (111, 112), 113, 21), W.product_gl(14, (114, 115), 22),
              W.product_bb(13, (116, 117, 118), 23), W.product_bb(12, (119, 120, 121, 122), 24),
              W.zerocheck_bb4(13, (123, 124, 125), 126, 25), W.product_bb_ext(12, (127,), 26),
              W.zerocheck_bb4(14, (128, 129), 130, 27)]
     insts[6].transcript_kind = "blake3"
     for inst in insts:
-        ref = _prove_with_tune(inst, tail_pairs=tail_pairs, lookahead=0)
-        got = _prove_with_tune(inst, tail_pairs=tail_pairs, lookahead=1)
+        ref = _prove_with_tune(inst, **tune, lookahead=0)
+        got = _prove_with_tune(inst, **tune, lookahead=1)
         assert got.reduction_proof.words.tolist() == ref.reduction_proof.words.tolist()
         assert got.reduced_claim.point.words.tolist() == ref.reduced_claim.point.words.tolist()
         assert got.reduced_claim.value.words.tolist() == ref.reduced_claim.value.words.tolist()

@@ -99,7 +99,8 @@ class InstanceImpl final : public Instance {
     o_point_ = take((size_t)n_ * sizeof(TC));
     o_final_ = take((size_t)(K + 1) * sizeof(TC));
     out_bytes_ = off;
-    const size_t o_slots = take((size_t)NSLOT * SLOTW * sizeof(uint64_t));
+    // Slot rows of the pass and chained kernels, then the look-ahead chain's.
+    const size_t o_slots = take(((size_t)NSLOT * SLOTW + LA_SLOT_WORDS) * sizeof(uint64_t));
     size_t o_eqp = 0, o_eqab = 0, o_eql = 0, o_eqr = 0;
     if (EQ) {
       o_eqp = take((size_t)n_ * sizeof(TC));
@@ -205,7 +206,8 @@ class InstanceImpl final : public Instance {
         // Chained: rounds r .. (tail or last) in one persistent launch.
         uint32_t end = r;
         while (end < last && ((size_t)1 << (n_ - end - 1)) > tune_.tail_pairs) ++end;
-        int rc = enqueue_chain(r, end, last, s);
+        int rc = tune_.lookahead && end - r >= 2 ? enqueue_chain_la(r, end, last, s)
+                                                 : enqueue_chain(r, end, last, s);
         if (rc) return rc;
         r = end - 1;
         continue;
@@ -267,6 +269,40 @@ class InstanceImpl final : public Instance {
     while (grid & (grid - 1)) grid &= grid - 1;
     hipEvent_t ev = begin(s);
     kern<<<(uint32_t)grid, PASS_THREADS, 0, s>>>(args_, first, end, last);
+    ZS_HIP(hipGetLastError());
+    if (ev) {
+      uint64_t b = 0;
+      for (uint32_t q = first; q < end; ++q) b += pass_bytes(q);
+      mark(s, first, KREC_CHAIN, (uint32_t)grid, b, ev);
+    }
+    return ZS_OK;
+  }
+
+  // Rounds [first, end) (at least two) in one launch of chain_la_kernel:
+  // block 0 finishes, and blocks 1..grid, sized as in enqueue_chain from the
+  // first period's pairs (round first+1's), stream the periods.
+  int enqueue_chain_la(uint32_t first, uint32_t end, uint32_t last, hipStream_t s) {
+    static int resident = 0;  // per instantiation: blocks of this kernel the device holds
+    auto kern = chain_la_kernel<FT, FC, K, EQ>;
+    if (resident == 0) {
+      int per_cu = 0, dev = 0, cus = 0;
+      ZS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PASS_THREADS, 0));
+      ZS_HIP(hipGetDevice(&dev));
+      ZS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      resident = per_cu * cus;
+      if (resident < 2) return fail(ZS_EHIP, "chained rounds: kernel does not fit on the device");
+    }
+    const size_t units = (size_t)1 << (n_ - first - 2);
+    const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
+    size_t grid = (units + per_block - 1) / per_block;
+    uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
+    if (grid > maxg) grid = maxg;
+    if (grid > (size_t)resident / 2) grid = (size_t)resident / 2;
+    if (grid > tune_.chain_grid) grid = tune_.chain_grid;
+    if (grid < 1) grid = 1;
+    while (grid & (grid - 1)) grid &= grid - 1;
+    hipEvent_t ev = begin(s);
+    kern<<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_, first, end, last);
     ZS_HIP(hipGetLastError());
     if (ev) {
       uint64_t b = 0;

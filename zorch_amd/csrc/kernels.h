@@ -1526,8 +1526,9 @@ __device__ __forceinline__ void la_poly(const typename FC::T (&A)[K], const type
 // Parts of a period: fold round k-1's tables at cm.r into round k's and sum
 // round k's node values into d (LA_FOLD_D); sum the coefficients of round
 // k+1's node sums into c[t * (K+1) + i] from round k's tables (LA_C); both
-// fold and coefficients (LA_FOLD_C; the fold alone when k = n-1).
-constexpr int LA_FOLD_D = 0, LA_C = 1, LA_FOLD_C = 2;
+// fold and coefficients (LA_FOLD_C); the fold alone (LA_FOLD, a thread per
+// group of four entries, or per entry below four).
+constexpr int LA_FOLD_D = 0, LA_C = 1, LA_FOLD_C = 2, LA_FOLD = 3;
 
 // Period P_k (part MODE) for thread wt of nt.
 template <class FS, class FC, int K, bool EQ, int MODE>
@@ -1537,13 +1538,18 @@ __device__ __forceinline__ void la_period(const StageArgs<FS, FC>& a, uint32_t k
                                           typename FC::T (&c)[(K + 1) * (K + 1)]) {
   using T = typename FC::T;
   const uint32_t n = a.n;
-  if (MODE == LA_FOLD_C && k + 1 >= n) {
-    // Round n-1's two entries per table from round n-2's four.
-    if (wt < 2u * K) {
-      T* s = a.work + (size_t)(wt >> 1) * a.work_stride;
-      const uint32_t y = wt & 1u;
-      const T lo = s[y], hi = s[y + 2];
-      s[y] = FC::add(lo, cm.mul(FC::sub(hi, lo)));
+  if constexpr (MODE == LA_FOLD) {
+    const uint32_t L = 1u << (n - k), per = L >= 4u ? 4u : 1u, ng = L / per;
+    for (uint32_t g = wt; g < ng; g += nt) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        T* s = a.work + (size_t)q * a.work_stride;
+        for (uint32_t m = 0; m < per; ++m) {
+          const uint32_t y = g + m * ng;
+          const T lo = s[y], hi = s[y + L];
+          s[y] = FC::add(lo, cm.mul(FC::sub(hi, lo)));
+        }
+      }
     }
     return;
   }
@@ -1724,7 +1730,10 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
 #pragma unroll
       for (int i = 0; i < NC; ++i) c[i] = FC::zero();
       const uint32_t wt = (uint32_t)widx * 64u + lane;
-      la_period<FS, FC, K, EQ, LA_FOLD_C>(a, k, wt, LA_WAVES * 64u, cm, d, c);
+      if (k + 1 < n)
+        la_period<FS, FC, K, EQ, LA_FOLD_C>(a, k, wt, LA_WAVES * 64u, cm, d, c);
+      else
+        la_period<FS, FC, K, EQ, LA_FOLD>(a, k, wt, LA_WAVES * 64u, cm, d, c);
       const uint32_t q8 = k + 2 <= n ? 1u << (n - k - 2) : 0u;
       la_arrive<FC, NC>(c, k + 1 < n && (uint32_t)widx * 64u < q8,
                         sh.part[k & 1][widx] + NA, &sh.arrived);
@@ -1797,6 +1806,232 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   }
   if (EQ && a.final_out && lane == 0) a.final_out[K] = sh.sm.eq_prev;
   fin_store(sh.sm, a.ctl);
+}
+
+// ------------------------------------------------ look-ahead chained rounds
+// Rounds [first, last) (first >= 2, at least two rounds) of one proof in one
+// persistent launch, with the periods of the look-ahead tail spread over
+// blocks 1..G (all resident) and block 0 as the finisher: period P_k folds
+// round k-1's tables into round k's and sums the coefficients C_{k+1} of
+// round k+1's node sums (polynomials in r_k) while block 0 hashes round k's
+// message, which it forms from C_k and r_{k-1}. The first period folds and
+// sums round `first`'s node values directly, then sums C_{first+1} from the
+// folded tables while that message is hashed. The period of round last-1
+// only folds: the next launch (or, when last = n, block 0's final fold)
+// reads round last-1's tables. Cross-block sums go through two parities of
+// slot rows (the coefficients of period k land in parity k & 1, read and
+// cleared by block 0 before it lets period k+2 start); every block that works
+// in a period arrives at the control block's counter (release), and block 0
+// publishes r_k (`done`, release) once every block has arrived for period k,
+// whose tables period k+1 reads. The claim is reduced at round
+// claim_last - 1 only. Waits are bounded as in chain_kernel.
+constexpr int LA_NSLOT = 8;     // slot rows per parity
+constexpr int LA_SLOTW = 128;   // 64-bit words per row: round sums, then coefficients
+constexpr size_t LA_SLOT_WORDS = (size_t)2 * LA_NSLOT * LA_SLOTW;
+
+// Blocks of a period with `units` thread units of work: one per PASS_THREADS,
+// at least one, at most grid.
+__device__ __forceinline__ uint32_t la_blocks(uint32_t units, uint32_t grid) {
+  const uint32_t b = units / PASS_THREADS;
+  return b < 1u ? 1u : b < grid ? b : grid;
+}
+
+// Thread units of period k of [first, last) (n rounds): round k+1's pairs,
+// or for the fold-only period the fold's groups of four entries.
+__device__ __forceinline__ uint32_t la_units(uint32_t n, uint32_t k, uint32_t last) {
+  if (k + 1 < last) return 1u << (n - k - 2);
+  const uint32_t L = 1u << (n - k);
+  return L >= 4u ? L / 4u : L;
+}
+
+template <class FC, int NV>
+__device__ __forceinline__ void la_slot_add(uint64_t* rows, uint32_t blk, uint32_t word_off,
+                                            const typename FC::T* vals) {
+  constexpr int W = SlotOps<FC>::W;
+  static_assert(NV * W <= LA_SLOTW, "look-ahead slot row width");
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(NV * W); i += PASS_THREADS) {
+    uint64_t* slot = rows + (size_t)(blk % LA_NSLOT) * LA_SLOTW + word_off + i;
+    __hip_atomic_fetch_add(slot, (uint64_t)SlotOps<FC>::word(vals[i / W], (int)(i % W)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Sum (and clear) words [word_off, word_off + NV W) of the rows into out[0..NV).
+// Whole block; ends with a barrier.
+template <class FC, int NV>
+__device__ __forceinline__ void la_slot_sum(uint64_t* rows, uint32_t word_off,
+                                            uint64_t (*sum64)[LA_SLOTW], typename FC::T* out) {
+  constexpr int W = SlotOps<FC>::W;
+  constexpr int G = PASS_THREADS / LA_SLOTW;
+  static_assert(LA_NSLOT % G == 0, "look-ahead slot layout");
+  const uint32_t w = threadIdx.x % LA_SLOTW, g = threadIdx.x / LA_SLOTW;
+  uint64_t s = 0;
+  if (w < (uint32_t)(NV * W)) {
+#pragma unroll
+    for (int i = 0; i < LA_NSLOT / G; ++i) {
+      uint64_t* p = rows + (size_t)(g + i * G) * LA_SLOTW + word_off + w;
+      s += __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(p, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  sum64[g][w] = s;
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)NV) {
+    uint64_t tot[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      uint64_t v = 0;
+#pragma unroll
+      for (int q = 0; q < G; ++q) v += sum64[q][threadIdx.x * W + c];
+      tot[c] = v;
+    }
+    out[threadIdx.x] = SlotOps<FC>::join(tot);
+  }
+  __syncthreads();
+}
+
+template <class FC, int K>
+struct ChainLaLds {
+  static constexpr int NA = K + 1, NC = NA * NA;
+  FinishLds<FC> sm;
+  typename FC::T lds[(PASS_THREADS / 64) * NC];
+  typename FC::T out[NC];
+  uint64_t sum64[PASS_THREADS / LA_SLOTW][LA_SLOTW];
+  typename FC::T r;
+  uint32_t ok;
+};
+
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
+    StageArgs<FT, FC> a, uint32_t first, uint32_t last, uint32_t claim_last) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1, NC = NA * NA;
+  constexpr int W = SlotOps<FC>::W;
+  __shared__ ChainLaLds<FC, K> sh;
+  const uint32_t tid = threadIdx.x, n = a.n;
+  const uint32_t G = gridDim.x - 1;
+  Ctl<FC>* ctl = a.ctl;
+  uint64_t* const rows0 = a.slots + (size_t)NSLOT * SLOTW;  // parity 0, then parity 1
+  auto rows = [&](uint32_t k) { return rows0 + (size_t)(k & 1u) * LA_NSLOT * LA_SLOTW; };
+  const uint32_t nbA = la_blocks(1u << (n - first - 2), G);
+  if (blockIdx.x == 0) {
+    // The finisher.
+    if (tid < 64) fin_load(sh.sm, ctl);
+    if (tid == 0) trace_mark(a.trace, first, 7);
+    uint32_t target = nbA;
+    for (uint32_t k = first; k < last; ++k) {
+      if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, k);
+      if (k == first) {
+        if (tid == 0) sh.ok = chain_wait(&ctl->counter, target, &ctl->fault) ? 1u : 0u;
+        __syncthreads();
+        if (!sh.ok) return;
+        la_slot_sum<FC, NA>(rows(first), 0, sh.sum64, sh.out);
+      } else {
+        // Round k's node sums: period k-1's coefficients at r_{k-1}.
+        la_slot_sum<FC, NC>(rows(k - 1), NA * W, sh.sum64, sh.out);
+        if (tid < (uint32_t)NA) {
+          const T r = sh.sm.r;
+          T v = sh.out[tid * NA + K];
+#pragma unroll
+          for (int i = K - 1; i >= 0; --i) v = FC::add(sh.out[tid * NA + i], FC::mul(r, v));
+          sh.lds[tid] = v;
+        }
+        __syncthreads();
+        if (tid < (uint32_t)NA) sh.out[tid] = sh.lds[tid];
+        __syncthreads();
+      }
+      if (tid == 0) trace_mark(a.trace, k, 2);
+      if (tid < 64) {
+        StageArgs<FT, FC> af = a;
+        af.final_out = nullptr;  // the final fold waits for the last period
+        finish_round<FT, FC, K, EQ, HOP_CHAIN>(af, k, k + 1 == claim_last, sh.out, sh.sm,
+                                               a.work, a.in, a.work_stride, false);
+      }
+      // Period k complete: publish r_k, which starts period k+1.
+      target += k == first ? nbA : la_blocks(la_units(n, k, last), G);
+      if (tid == 0) {
+        bool ok = chain_wait(&ctl->counter, target, &ctl->fault);
+        if (ok && k + 1 < last) {
+          ctl->r = sh.sm.r;
+          __hip_atomic_store(&ctl->done, k - first + 1, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
+        sh.ok = ok ? 1u : 0u;
+      }
+      __syncthreads();
+      if (!sh.ok) return;
+    }
+    if (tid < 64) {
+      if (last == n && a.final_out && tid < (uint32_t)K) {
+        const T* s = a.work + (size_t)tid * a.work_stride;
+        const T lo = s[0], hi = s[1];
+        a.final_out[tid] = FC::add(lo, FC::mul(sh.sm.r, FC::sub(hi, lo)));
+      }
+      if (EQ && last == n && a.final_out && tid == 0) a.final_out[K] = sh.sm.eq_prev;
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) {
+      __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  if (blk >= nbA) return;  // the first period has the most work
+  auto arrive = [&]() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  {
+    // The first period: fold and round first's node sums ...
+    ChalMul<FC> cm;
+    cm.init(ctl->r);
+    T d[NA], c[NC];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) d[t] = FC::zero();
+    la_period<FT, FC, K, EQ, LA_FOLD_D>(a, first, blk * PASS_THREADS + tid, nbA * PASS_THREADS,
+                                        cm, d, c);
+    block_reduce<FC, NA, PASS_THREADS>(d, sh.lds, sh.out);
+    la_slot_add<FC, NA>(rows(first), blk, 0, sh.out);
+    arrive();
+    // ... then, once every block has folded, round first+1's coefficients.
+    if (tid == 0) sh.ok = chain_wait(&ctl->counter, nbA, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.ok) return;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c[i] = FC::zero();
+    la_period<FT, FC, K, EQ, LA_C>(a, first, blk * PASS_THREADS + tid, nbA * PASS_THREADS, cm,
+                                   d, c);
+    block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
+    la_slot_add<FC, NC>(rows(first), blk, NA * W, sh.out);
+    arrive();
+  }
+  for (uint32_t k = first + 1; k < last; ++k) {
+    const uint32_t nb = la_blocks(la_units(n, k, last), G);
+    if (blk >= nb) return;
+    if (tid == 0) {
+      const bool ok = chain_wait(&ctl->done, k - first, &ctl->fault);
+      if (ok) sh.r = ctl->r;
+      sh.ok = ok ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!sh.ok) return;
+    ChalMul<FC> cm;
+    cm.init(sh.r);
+    T d[NA], c[NC];
+    const uint32_t wt = blk * PASS_THREADS + tid, nt = nb * PASS_THREADS;
+    if (k + 1 < last) {
+#pragma unroll
+      for (int i = 0; i < NC; ++i) c[i] = FC::zero();
+      la_period<FT, FC, K, EQ, LA_FOLD_C>(a, k, wt, nt, cm, d, c);
+      block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
+      la_slot_add<FC, NC>(rows(k), blk, NA * W, sh.out);
+    } else {
+      la_period<FT, FC, K, EQ, LA_FOLD>(a, k, wt, nt, cm, d, c);
+    }
+    arrive();
+  }
 }
 
 // ------------------------------------------------------------ batched steps
