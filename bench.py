@@ -48,6 +48,9 @@ def build_instance(name: str):
 FIELD_CLASS = {"babybear_mont": "zs::BB", "babybearx4_mont": "zs::BB4", "goldilocks": "zs::GL"}
 
 
+KERNEL_KINDS = ("pass", "chain", "tail", "pass_la", "chain_la", "tail_la")
+
+
 def kernel_symbol(rec: dict, inst, eq: bool) -> str:
     """Demangled kernel name of a timing record, as rocprofv3 lists it."""
     ft = FIELD_CLASS[inst.tables.field.name]
@@ -57,6 +60,12 @@ def kernel_symbol(rec: dict, inst, eq: bool) -> str:
         return f"void zs::tail_kernel<{ft}, {fc}, {k}, {e}>"
     if rec["kind"] == "chain":
         return f"void zs::chain_kernel<{ft}, {fc}, {k}, {e}>"
+    if rec["kind"] == "tail_la":
+        return f"void zs::tail_la_kernel<{ft}, {fc}, {k}, {e}>"
+    if rec["kind"] == "chain_la":
+        return f"void zs::chain_la_kernel<{ft}, {fc}, {k}, {e}>"
+    if rec["kind"] == "pass_la":
+        return f"void zs::pass_la_kernel<{ft}, {fc}, {k}, {e}>"
     if rec["round"] == 0:
         return f"void zs::pass_kernel<{ft}, {ft}, {fc}, {k}, {e}, false>"
     if rec["round"] == 1:
@@ -68,7 +77,7 @@ def kernel_groups(recs: list, inst, eq: bool):
     """Per-round means and per-symbol totals of the pass, chain and tail records."""
     passes: dict = {}
     for r in recs:
-        if r["kind"] in ("pass", "chain", "tail"):
+        if r["kind"] in KERNEL_KINDS:
             passes.setdefault((r["kind"], r["round"]), []).append(r)
     nproofs = max(len(v) for v in passes.values())
     per_round = []
@@ -79,7 +88,7 @@ def kernel_groups(recs: list, inst, eq: bool):
                           "gbs": round(rs[0]["bytes"] / ms / 1e6, 1)})
     by_sym: dict = {}
     for r in recs:
-        if r["kind"] in ("pass", "chain", "tail"):
+        if r["kind"] in KERNEL_KINDS:
             g = by_sym.setdefault(kernel_symbol(r, inst, eq), {"ms": 0.0, "bytes": 0, "n": 0})
             g["ms"] += r["ms"]
             g["bytes"] += r["bytes"]

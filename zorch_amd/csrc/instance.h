@@ -185,12 +185,24 @@ class InstanceImpl final : public Instance {
     return e;
   }
 
+  // Whether round r of [.., last) starts a look-ahead chain in enqueue.
+  bool chain_la_starts(uint32_t r, uint32_t last) const {
+    if (!tune_.lookahead || r < 2 || r >= last) return false;
+    const size_t pairs = (size_t)1 << (n_ - r - 1);
+    if (pairs <= tune_.tail_pairs || pairs > tune_.chain_pairs) return false;
+    uint32_t end = r;
+    while (end < last && ((size_t)1 << (n_ - end - 1)) > tune_.tail_pairs) ++end;
+    return end - r >= 2;
+  }
+
   int enqueue(uint32_t first, uint32_t last, hipStream_t s) override {
+    bool cont = false;  // the previous launch left this round's coefficients
     for (uint32_t r = first; r < last; ++r) {
       const size_t pairs = (size_t)1 << (n_ - r - 1);
       if (pairs <= tune_.tail_pairs) {
         hipEvent_t ev = begin(s);
-        if (tune_.lookahead && r >= 2 && last == n_ && n_ - r >= 2)
+        const bool la = tune_.lookahead && r >= 2 && last == n_ && n_ - r >= 2;
+        if (la)
           tail_la_kernel<FT, FC, K, EQ><<<1, LA_THREADS, 0, s>>>(args_, r);
         else
           tail_kernel<FT, FC, K, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
@@ -198,7 +210,7 @@ class InstanceImpl final : public Instance {
         if (ev) {
           uint64_t b = 0;
           for (uint32_t q = r; q < last; ++q) b += pass_bytes(q);
-          mark(s, r, KREC_TAIL, 1, b, ev);
+          mark(s, r, la ? KREC_TAIL_LA : KREC_TAIL, 1, b, ev);
         }
         return ZS_OK;
       }
@@ -206,9 +218,10 @@ class InstanceImpl final : public Instance {
         // Chained: rounds r .. (tail or last) in one persistent launch.
         uint32_t end = r;
         while (end < last && ((size_t)1 << (n_ - end - 1)) > tune_.tail_pairs) ++end;
-        int rc = tune_.lookahead && end - r >= 2 ? enqueue_chain_la(r, end, last, s)
-                                                 : enqueue_chain(r, end, last, s);
+        const bool la = tune_.lookahead && end - r >= 2;
+        int rc = la ? enqueue_chain_la(r, end, last, cont, s) : enqueue_chain(r, end, last, s);
         if (rc) return rc;
+        cont = false;
         r = end - 1;
         continue;
       }
@@ -224,6 +237,15 @@ class InstanceImpl final : public Instance {
       hipEvent_t ev = begin(s);
       // Block 0 of a pass kernel finishes the round; grid blocks stream it.
       const uint32_t blocks = (uint32_t)grid + 1;
+      if (r >= 2 && chain_la_starts(r + 1, last)) {
+        // The round before a look-ahead chain also sums the chain's first
+        // coefficients, so that the chain hashes its first message at once.
+        pass_la_kernel<FT, FC, K, EQ><<<blocks, PASS_THREADS, 0, s>>>(args_, r);
+        ZS_HIP(hipGetLastError());
+        if (ev) mark(s, r, KREC_PASS_LA, (uint32_t)grid, pass_bytes(r), ev);
+        cont = true;
+        continue;
+      }
       if (r == 0) {
         pass_kernel<FT, FT, FC, K, EQ, false>
             <<<blocks, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
@@ -280,8 +302,9 @@ class InstanceImpl final : public Instance {
 
   // Rounds [first, end) (at least two) in one launch of chain_la_kernel:
   // block 0 finishes, and blocks 1..grid, sized as in enqueue_chain from the
-  // first period's pairs (round first+1's), stream the periods.
-  int enqueue_chain_la(uint32_t first, uint32_t end, uint32_t last, hipStream_t s) {
+  // first period's pairs (round first+1's), stream the periods. `cont`: the
+  // previous launch (pass_la_kernel) left round first's coefficients.
+  int enqueue_chain_la(uint32_t first, uint32_t end, uint32_t last, bool cont, hipStream_t s) {
     static int resident = 0;  // per instantiation: blocks of this kernel the device holds
     auto kern = chain_la_kernel<FT, FC, K, EQ>;
     if (resident == 0) {
@@ -302,12 +325,12 @@ class InstanceImpl final : public Instance {
     if (grid < 1) grid = 1;
     while (grid & (grid - 1)) grid &= grid - 1;
     hipEvent_t ev = begin(s);
-    kern<<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_, first, end, last);
+    kern<<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_, first, end, last, cont ? 1u : 0u);
     ZS_HIP(hipGetLastError());
     if (ev) {
       uint64_t b = 0;
       for (uint32_t q = first; q < end; ++q) b += pass_bytes(q);
-      mark(s, first, KREC_CHAIN, (uint32_t)grid, b, ev);
+      mark(s, first, KREC_CHAIN_LA, (uint32_t)grid, b, ev);
     }
     return ZS_OK;
   }

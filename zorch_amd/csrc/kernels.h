@@ -871,11 +871,11 @@ __device__ __forceinline__ void slot_row_add(uint64_t* slots, uint32_t blk,
   }
 }
 
-// Sum of the slot rows into sh.out (NA elements), which it clears for the
-// next round; ends with a barrier. Every block that added to the rows has
-// completed its additions.
-template <class FC, int NA>
-__device__ __forceinline__ void slot_rows_sum(uint64_t* slots, RoundLds<FC, NA>& sh) {
+// Sum of the slot rows into sh.out (NA elements; sh a RoundLds), which it
+// clears for the next round; ends with a barrier. Every block that added to
+// the rows has completed its additions.
+template <class FC, int NA, class Lds>
+__device__ __forceinline__ void slot_rows_sum(uint64_t* slots, Lds& sh) {
   constexpr int W = SlotOps<FC>::W;
   constexpr int NW = NA * W;
   constexpr int G = PASS_THREADS / SLOTW;
@@ -1527,8 +1527,9 @@ __device__ __forceinline__ void la_poly(const typename FC::T (&A)[K], const type
 // round k's node values into d (LA_FOLD_D); sum the coefficients of round
 // k+1's node sums into c[t * (K+1) + i] from round k's tables (LA_C); both
 // fold and coefficients (LA_FOLD_C); the fold alone (LA_FOLD, a thread per
-// group of four entries, or per entry below four).
-constexpr int LA_FOLD_D = 0, LA_C = 1, LA_FOLD_C = 2, LA_FOLD = 3;
+// group of four entries, or per entry below four); fold, round sums and
+// coefficients (LA_FOLD_DC).
+constexpr int LA_FOLD_D = 0, LA_C = 1, LA_FOLD_C = 2, LA_FOLD = 3, LA_FOLD_DC = 4;
 
 // Period P_k (part MODE) for thread wt of nt.
 template <class FS, class FC, int K, bool EQ, int MODE>
@@ -1582,7 +1583,7 @@ __device__ __forceinline__ void la_period(const StageArgs<FS, FC>& a, uint32_t k
         }
       }
     }
-    if constexpr (MODE == LA_FOLD_D) {
+    if constexpr (MODE == LA_FOLD_D || MODE == LA_FOLD_DC) {
       // Round k's pairs g = (e0, e2) and g + L/8 = (e1, e3).
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1610,7 +1611,7 @@ __device__ __forceinline__ void la_period(const StageArgs<FS, FC>& a, uint32_t k
           }
         }
       }
-      continue;
+      if constexpr (MODE == LA_FOLD_D) continue;
     }
     // Round k+1's pair g: lo = a0 + r b0, hi = a1 + r b1 per table.
     T A[K], B[K], dA[K], dB[K];
@@ -1829,6 +1830,12 @@ constexpr int LA_NSLOT = 8;     // slot rows per parity
 constexpr int LA_SLOTW = 128;   // 64-bit words per row: round sums, then coefficients
 constexpr size_t LA_SLOT_WORDS = (size_t)2 * LA_NSLOT * LA_SLOTW;
 
+// Look-ahead rows of the sums of round x (parity x & 1): after the pass and
+// chained kernels' slot rows.
+__device__ __forceinline__ uint64_t* la_rows(uint64_t* slots, uint32_t x) {
+  return slots + (size_t)NSLOT * SLOTW + (size_t)(x & 1u) * LA_NSLOT * LA_SLOTW;
+}
+
 // Blocks of a period with `units` thread units of work: one per PASS_THREADS,
 // at least one, at most grid.
 __device__ __forceinline__ uint32_t la_blocks(uint32_t units, uint32_t grid) {
@@ -1903,7 +1910,7 @@ struct ChainLaLds {
 
 template <class FT, class FC, int K, bool EQ>
 __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
-    StageArgs<FT, FC> a, uint32_t first, uint32_t last, uint32_t claim_last) {
+    StageArgs<FT, FC> a, uint32_t first, uint32_t last, uint32_t claim_last, uint32_t cont) {
   using T = typename FC::T;
   constexpr int NA = K + 1, NC = NA * NA;
   constexpr int W = SlotOps<FC>::W;
@@ -1911,24 +1918,25 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
   const uint32_t tid = threadIdx.x, n = a.n;
   const uint32_t G = gridDim.x - 1;
   Ctl<FC>* ctl = a.ctl;
-  uint64_t* const rows0 = a.slots + (size_t)NSLOT * SLOTW;  // parity 0, then parity 1
-  auto rows = [&](uint32_t k) { return rows0 + (size_t)(k & 1u) * LA_NSLOT * LA_SLOTW; };
-  const uint32_t nbA = la_blocks(1u << (n - first - 2), G);
+  // Blocks of the first period (the busiest): with `cont`, P_first folds and
+  // sums coefficients; else its fold-and-sums part, then its coefficients.
+  const uint32_t nb0 = la_blocks(cont ? la_units(n, first, last) : 1u << (n - first - 2), G);
   if (blockIdx.x == 0) {
     // The finisher.
     if (tid < 64) fin_load(sh.sm, ctl);
     if (tid == 0) trace_mark(a.trace, first, 7);
-    uint32_t target = nbA;
+    uint32_t target = cont ? 0u : nb0;
     for (uint32_t k = first; k < last; ++k) {
       if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, k);
-      if (k == first) {
+      if (k == first && !cont) {
         if (tid == 0) sh.ok = chain_wait(&ctl->counter, target, &ctl->fault) ? 1u : 0u;
         __syncthreads();
         if (!sh.ok) return;
-        la_slot_sum<FC, NA>(rows(first), 0, sh.sum64, sh.out);
+        la_slot_sum<FC, NA>(la_rows(a.slots, first), 0, sh.sum64, sh.out);
       } else {
-        // Round k's node sums: period k-1's coefficients at r_{k-1}.
-        la_slot_sum<FC, NC>(rows(k - 1), NA * W, sh.sum64, sh.out);
+        // Round k's node sums: its coefficients (from the previous period,
+        // or with `cont` the previous launch) at r_{k-1}.
+        la_slot_sum<FC, NC>(la_rows(a.slots, k), NA * W, sh.sum64, sh.out);
         if (tid < (uint32_t)NA) {
           const T r = sh.sm.r;
           T v = sh.out[tid * NA + K];
@@ -1948,7 +1956,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
                                                a.work, a.in, a.work_stride, false);
       }
       // Period k complete: publish r_k, which starts period k+1.
-      target += k == first ? nbA : la_blocks(la_units(n, k, last), G);
+      target += k == first ? nb0 : la_blocks(la_units(n, k, last), G);
       if (tid == 0) {
         bool ok = chain_wait(&ctl->counter, target, &ctl->fault);
         if (ok && k + 1 < last) {
@@ -1977,37 +1985,37 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
     return;
   }
   const uint32_t blk = blockIdx.x - 1;
-  if (blk >= nbA) return;  // the first period has the most work
+  if (blk >= nb0) return;  // the first period has the most work
   auto arrive = [&]() {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   };
-  {
+  if (!cont) {
     // The first period: fold and round first's node sums ...
     ChalMul<FC> cm;
     cm.init(ctl->r);
     T d[NA], c[NC];
 #pragma unroll
     for (int t = 0; t < NA; ++t) d[t] = FC::zero();
-    la_period<FT, FC, K, EQ, LA_FOLD_D>(a, first, blk * PASS_THREADS + tid, nbA * PASS_THREADS,
+    la_period<FT, FC, K, EQ, LA_FOLD_D>(a, first, blk * PASS_THREADS + tid, nb0 * PASS_THREADS,
                                         cm, d, c);
     block_reduce<FC, NA, PASS_THREADS>(d, sh.lds, sh.out);
-    la_slot_add<FC, NA>(rows(first), blk, 0, sh.out);
+    la_slot_add<FC, NA>(la_rows(a.slots, first), blk, 0, sh.out);
     arrive();
     // ... then, once every block has folded, round first+1's coefficients.
-    if (tid == 0) sh.ok = chain_wait(&ctl->counter, nbA, &ctl->fault) ? 1u : 0u;
+    if (tid == 0) sh.ok = chain_wait(&ctl->counter, nb0, &ctl->fault) ? 1u : 0u;
     __syncthreads();
     if (!sh.ok) return;
 #pragma unroll
     for (int i = 0; i < NC; ++i) c[i] = FC::zero();
-    la_period<FT, FC, K, EQ, LA_C>(a, first, blk * PASS_THREADS + tid, nbA * PASS_THREADS, cm,
+    la_period<FT, FC, K, EQ, LA_C>(a, first, blk * PASS_THREADS + tid, nb0 * PASS_THREADS, cm,
                                    d, c);
     block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
-    la_slot_add<FC, NC>(rows(first), blk, NA * W, sh.out);
+    la_slot_add<FC, NC>(la_rows(a.slots, first + 1), blk, NA * W, sh.out);
     arrive();
   }
-  for (uint32_t k = first + 1; k < last; ++k) {
+  for (uint32_t k = cont ? first : first + 1; k < last; ++k) {
     const uint32_t nb = la_blocks(la_units(n, k, last), G);
     if (blk >= nb) return;
     if (tid == 0) {
@@ -2026,12 +2034,75 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
       for (int i = 0; i < NC; ++i) c[i] = FC::zero();
       la_period<FT, FC, K, EQ, LA_FOLD_C>(a, k, wt, nt, cm, d, c);
       block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
-      la_slot_add<FC, NC>(rows(k), blk, NA * W, sh.out);
+      la_slot_add<FC, NC>(la_rows(a.slots, k + 1), blk, NA * W, sh.out);
     } else {
       la_period<FT, FC, K, EQ, LA_FOLD>(a, k, wt, nt, cm, d, c);
     }
     arrive();
   }
+}
+
+// Round k >= 2 of one proof as 1 + nblk blocks, starting the look-ahead:
+// blocks 1..nblk fold round k-1's tables, sum round k's node values into the
+// slot rows and round k+1's coefficients into the look-ahead rows of round
+// k+1, and arrive; block 0 finishes round k from the node sums as in
+// pass_kernel. Round k+1 (k + 1 < n) starts a look-ahead chain with `cont`.
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_la_kernel(
+    StageArgs<FT, FC> a, uint32_t round) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1, NC = NA * NA;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  constexpr int W = SlotOps<FC>::W;
+  __shared__ PassLds<FC, NC> ps;
+  RoundLds<FC, NC>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
+  Ctl<FC>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, round);
+      fin_load(sh.sm, ctl);
+      // Warm-up hop on a copy of the stream state, as in pass_kernel.
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    slot_rows_sum<FC, NA>(a.slots, sh);
+    if (tid < 64) {
+      StageArgs<FT, FC> af = a;
+      af.final_out = nullptr;
+      finish_round<FT, FC, K, EQ, HOP_PASS>(af, round, false, sh.out, sh.sm, a.work, a.in,
+                                            a.work_stride, false);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  ChalMul<FC> cm;
+  cm.init(ctl->r);
+  T d[NA], c[NC];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) d[t] = FC::zero();
+#pragma unroll
+  for (int i = 0; i < NC; ++i) c[i] = FC::zero();
+  la_period<FT, FC, K, EQ, LA_FOLD_DC>(a, round, blk * PASS_THREADS + tid, nblk * PASS_THREADS,
+                                       cm, d, c);
+  block_reduce<FC, NA, PASS_THREADS>(d, sh.lds, sh.out);
+  slot_row_add<FC, NA>(a.slots, blk, sh.out);
+  block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
+  la_slot_add<FC, NC>(la_rows(a.slots, round + 1), blk, NA * W, sh.out);
+  // The additions are complete before the arrival; the folded tables and the
+  // coefficients reach the next launch through the kernel boundary.
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------ batched steps

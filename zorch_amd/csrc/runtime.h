@@ -79,7 +79,10 @@ struct KernelRecord {
   uint64_t bytes;
   hipEvent_t start, stop;
 };
-enum { KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5 };
+enum {
+  KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5,
+  KREC_PASS_LA = 6, KREC_CHAIN_LA = 7, KREC_TAIL_LA = 8  // look-ahead variants
+};
 
 }  // namespace zs
 
