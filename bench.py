@@ -48,7 +48,7 @@ def build_instance(name: str):
 FIELD_CLASS = {"babybear_mont": "zs::BB", "babybearx4_mont": "zs::BB4", "goldilocks": "zs::GL"}
 
 
-KERNEL_KINDS = ("pass", "chain", "tail", "pass_la", "chain_la", "tail_la")
+KERNEL_KINDS = ("pass", "chain", "tail", "pass_la", "chain_la", "tail_la", "fused01", "fold2")
 
 
 def kernel_symbol(rec: dict, inst, eq: bool) -> str:
@@ -66,6 +66,10 @@ def kernel_symbol(rec: dict, inst, eq: bool) -> str:
         return f"void zs::chain_la_kernel<{ft}, {fc}, {k}, {e}>"
     if rec["kind"] == "pass_la":
         return f"void zs::pass_la_kernel<{ft}, {fc}, {k}, {e}>"
+    if rec["kind"] == "fused01":
+        return f"void zs::fused01_kernel<{ft}, {fc}, {k}, {e}>"
+    if rec["kind"] == "fold2":
+        return f"void zs::fold2_kernel<{ft}, {fc}, {k}, {e}>"
     if rec["round"] == 0:
         return f"void zs::pass_kernel<{ft}, {ft}, {fc}, {k}, {e}, false>"
     if rec["round"] == 1:

@@ -196,12 +196,17 @@ def test_chained_rounds_equal_one_launch_per_round(tune):
     dict(tail_pairs=0, chain_pairs=1 << 20),
     dict(tail_pairs=8, chain_pairs=1 << 20, max_grid=4),
     dict(tail_pairs=256, chain_pairs=1 << 12, pairs_per_thread=4),
+    # rounds 0-2 from the input (round 2 a pass round)
+    dict(tail_pairs=16, chain_pairs=0),
+    dict(tail_pairs=8, chain_pairs=64),
+    dict(tail_pairs=0, chain_pairs=0, max_grid=4),
 ])
 def test_lookahead_equals_plain_rounds(tune):
     """Tails and chained launches that fold each round while the previous
-    message is hashed equal the round-by-round kernels, bit for bit, over
-    fields, factor counts, both modes, both transcripts and every start and
-    end round; the C oracle anchors one of them."""
+    message is hashed, and rounds 0-2 computed from the input, equal the
+    round-by-round kernels, bit for bit, over fields, factor counts, both
+    modes, both transcripts and every start and end round; the C oracle
+    anchors one of them."""
     insts = [W.zerocheck_bb4(15, (111, 112), 113, 21), W.product_gl(14, (114, 115), 22),
              W.product_bb(13, (116, 117, 118), 23), W.product_bb(12, (119, 120, 121, 122), 24),
              W.zerocheck_bb4(13, (123, 124, 125), 126, 25), W.product_bb_ext(12, (127,), 26),

@@ -195,9 +195,45 @@ class InstanceImpl final : public Instance {
     return end - r >= 2;
   }
 
+  // Power-of-two grid for `units` thread units of work, as for a pass.
+  size_t pass_grid(size_t units) const {
+    const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
+    size_t grid = (units + per_block - 1) / per_block;
+    const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
+    if (grid > maxg) grid = maxg;
+    if (grid < 1) grid = 1;
+    while (grid & (grid - 1)) grid &= grid - 1;
+    return grid;
+  }
+
+  // Rounds 0-2 from the input (fused01_kernel, fold2_kernel) when a whole
+  // stage starts with three pass rounds and round 2 is not the last.
+  bool fused_first_rounds(uint32_t first, uint32_t last) const {
+    if (!tune_.lookahead || first != 0 || last < 3 || n_ < 4) return false;
+    const size_t pairs2 = (size_t)1 << (n_ - 3);
+    return pairs2 > tune_.tail_pairs && pairs2 > tune_.chain_pairs;
+  }
+
   int enqueue(uint32_t first, uint32_t last, hipStream_t s) override {
     bool cont = false;  // the previous launch left this round's coefficients
-    for (uint32_t r = first; r < last; ++r) {
+    uint32_t r0 = first;
+    if (fused_first_rounds(first, last)) {
+      const size_t N = (size_t)1 << n_;
+      const uint64_t in_bytes = (uint64_t)K * N * sizeof(TT);
+      size_t grid = pass_grid(N / 4);
+      hipEvent_t ev = begin(s);
+      fused01_kernel<FT, FC, K, EQ><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_);
+      ZS_HIP(hipGetLastError());
+      if (ev) mark(s, 0, KREC_FUSED01, (uint32_t)grid, in_bytes, ev);
+      grid = pass_grid(N / 8);
+      ev = begin(s);
+      fold2_kernel<FT, FC, K, EQ>
+          <<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_, last == 3 ? 1u : 0u);
+      ZS_HIP(hipGetLastError());
+      if (ev) mark(s, 2, KREC_FOLD2, (uint32_t)grid, in_bytes + (uint64_t)K * (N / 4) * sizeof(TC), ev);
+      r0 = 3;
+    }
+    for (uint32_t r = r0; r < last; ++r) {
       const size_t pairs = (size_t)1 << (n_ - r - 1);
       if (pairs <= tune_.tail_pairs) {
         hipEvent_t ev = begin(s);

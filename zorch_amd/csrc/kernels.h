@@ -2105,6 +2105,369 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void pass_la_kernel
   if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------ rounds 0-2 from the input
+// The first folded tables are never written. Round 1's tables are round 0's
+// folded at r_0, so round 1's node sums are polynomials of degree K in r_0
+// whose coefficients are sums of products of input entries (base-field work,
+// weighted by eq lazily): fused01_kernel streams the input once, sums round
+// 0's node values and round 1's coefficients, and its finisher block hashes
+// round 0's message, forms round 1's from the coefficients at r_0 and hashes
+// it too. fold2_kernel then reads the input again and folds it twice (at r_0
+// and r_1) into round 2's tables, which it writes and evaluates. Against
+// writing round 1's tables in the challenge field and reading them back,
+// this drops a write and a read of twice the input's size (BabyBear under
+// BabyBear^4) and one launch.
+#define FUSED_WAVES_PER_EU __attribute__((amdgpu_waves_per_eu(2, 8)))
+constexpr int HOP_FUSED = 2;
+
+// sum_m W[m] * x[m] over four terms (lazily for BabyBear^4 weights on
+// BabyBear values: four products per limb stay below 2 p 2^32).
+template <class FC, class FT>
+__device__ __forceinline__ typename FC::T mix4(const typename FC::T (&W)[4],
+                                               const typename FT::T (&x)[4]) {
+  typename FC::T s = MixMul<FC, FT>::go(W[0], x[0]);
+#pragma unroll
+  for (int m = 1; m < 4; ++m) s = FC::add(s, MixMul<FC, FT>::go(W[m], x[m]));
+  return s;
+}
+template <>
+__device__ __forceinline__ bb4 mix4<BB4, BB>(const bb4 (&W)[4], const uint32_t (&x)[4]) {
+  uint32_t o[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint64_t s = (uint64_t)W[0].c[c] * x[0];
+#pragma unroll
+    for (int m = 1; m < 4; ++m) s += (uint64_t)W[m].c[c] * x[m];
+    o[c] = bb_redc(bb_pre(s));
+  }
+  return bb4_make(o[0], o[1], o[2], o[3]);
+}
+
+// Eq weight of pair j of a round: with `fac`, the per-thread factor wr is
+// applied to the sums at the end and the pair takes wl alone.
+template <class FC>
+__device__ __forceinline__ typename FC::T eq_w(const typename FC::T* wl, const typename FC::T* wr,
+                                               uint32_t nx, bool fac, size_t j) {
+  if (fac) return wl[j >> nx];
+  if (wl) return FC::mul(wl[j >> nx], wr[j & (((size_t)1 << nx) - 1)]);
+  return wr[j];
+}
+
+template <class FC, int K>
+struct Fused01Lds {
+  static constexpr int NA = K + 1, NC = NA * NA;
+  RoundLds<FC, NC> round;
+  HopLds warm;
+};
+
+// Rounds 0 and 1 of one proof as 1 + nblk blocks (round 1 < n - 1).
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kernel(
+    StageArgs<FT, FC> a) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1, NC = NA * NA;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  constexpr int W = SlotOps<FC>::W;
+  __shared__ Fused01Lds<FC, K> ps;
+  RoundLds<FC, NC>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1, n = a.n;
+  Ctl<FC>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, 0);
+      fin_load(sh.sm, ctl);
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_FUSED>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                        (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    StageArgs<FT, FC> af = a;
+    af.final_out = nullptr;
+    slot_rows_sum<FC, NA>(a.slots, sh);
+    if (tid == 0) trace_mark(a.trace, 0, 2);
+    if (tid < 64)
+      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 0, false, sh.out, sh.sm, a.work, a.in,
+                                             a.in_stride, true);
+    // Round 1's node sums: its coefficients at r_0.
+    static_assert(sizeof(sh.sum64) == (PASS_THREADS / LA_SLOTW) * LA_SLOTW * sizeof(uint64_t),
+                  "slot scratch shapes");
+    la_slot_sum<FC, NC>(la_rows(a.slots, 1), NA * W,
+                        reinterpret_cast<uint64_t (*)[LA_SLOTW]>(&sh.sum64[0][0]), sh.out);
+    if (tid < (uint32_t)NA) {
+      const T r = sh.sm.r;
+      T v = sh.out[tid * NA + K];
+#pragma unroll
+      for (int i = K - 1; i >= 0; --i) v = FC::add(sh.out[tid * NA + i], FC::mul(r, v));
+      sh.lds[tid] = v;
+    }
+    __syncthreads();
+    if (tid < (uint32_t)NA) sh.out[tid] = sh.lds[tid];
+    if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, 1);
+    __syncthreads();
+    if (tid == 0) trace_mark(a.trace, 1, 2);
+    if (tid < 64) {
+      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 1, false, sh.out, sh.sm, a.work, a.in,
+                                             a.in_stride, true);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  const size_t q4 = (size_t)1 << (n - 2);  // round 1's pairs
+  const size_t start = (size_t)blk * PASS_THREADS + tid, stride = (size_t)nblk * PASS_THREADS;
+  const T* wl0 = nullptr;
+  const T* wr0 = nullptr;
+  const T* wl1 = nullptr;
+  const T* wr1 = nullptr;
+  uint32_t x0 = 0, x1 = 0;
+  bool fac = false;
+  if constexpr (EQ) {
+    select_eq<FC>(0, n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl0, wr0, x0);
+    select_eq<FC>(1, n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl1, wr1, x1);
+    const size_t mask = ((size_t)1 << x0) - 1;
+    fac = wl0 && wl1 && wr0 == wr1 && x0 == x1 && (stride & mask) == 0 && (q4 & mask) == 0;
+  }
+  WeightedSum<FC, FT> dsum[NA], csum[NA][NA];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) {
+    dsum[t].init();
+#pragma unroll
+    for (int i = 0; i < NA; ++i) csum[t][i].init();
+  }
+  // Group g: input entries g + m N/4 (m = 0..3) of every table: round 0's
+  // pairs g and g + N/4, round 1's pair g.
+  auto load = [&](size_t g, S (&x)[K][4]) {
+    g = g < q4 ? g : q4 - 1;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const S* s = a.in + (size_t)q * a.in_stride;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) x[q][m] = s[g + (size_t)m * q4];
+    }
+  };
+  S cur[K][4];
+  if (start < q4) load(start, cur);
+  uint32_t par = 0;
+  for (size_t g = start; g < q4; g += stride) {
+    T w0a = FC::one(), w0b = FC::one(), w1 = FC::one();
+    if constexpr (EQ) {
+      w0a = eq_w<FC>(wl0, wr0, x0, fac, g);
+      w0b = eq_w<FC>(wl0, wr0, x0, fac, g + q4);
+      w1 = eq_w<FC>(wl1, wr1, x1, fac, g);
+    }
+    S nxt[K][4];
+    load(g + stride, nxt);
+    const bool fold = (par++ & 1u) != 0;
+    // Round 0: pairs (x0, x2) and (x1, x3).
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      S lo[K], df[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        lo[q] = cur[q][h];
+        df[q] = FT::sub(cur[q][h + 2], cur[q][h]);
+      }
+#pragma unroll
+      for (int t = 0; t < NA; ++t) {
+        S prod = lo[0];
+#pragma unroll
+        for (int q = 1; q < K; ++q) prod = FT::mul(prod, lo[q]);
+        dsum[t].add(h ? w0b : w0a, prod, h == 1);
+        if (t < K) {
+#pragma unroll
+          for (int q = 0; q < K; ++q) lo[q] = FT::add(lo[q], df[q]);
+        }
+      }
+    }
+    // Round 1's pair g: lo = x0 + r (x2 - x0), hi = x1 + r (x3 - x1).
+    S A[K], B[K], dA[K], dB[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      A[q] = cur[q][0];
+      B[q] = FT::sub(cur[q][2], cur[q][0]);
+      dA[q] = FT::sub(cur[q][1], cur[q][0]);
+      dB[q] = FT::sub(FT::sub(cur[q][3], cur[q][1]), B[q]);
+    }
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      S p[K + 1];
+      la_poly<FT, K>(A, B, p);
+#pragma unroll
+      for (int i = 0; i < NA; ++i) csum[t][i].add(w1, p[i], fold);
+      if (t < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          A[q] = FT::add(A[q], dA[q]);
+          B[q] = FT::add(B[q], dB[q]);
+        }
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) cur[q][m] = nxt[q][m];
+  }
+  T d[NA], c[NC];
+  const T wrc = EQ && fac && start < q4 ? wr0[start & (((size_t)1 << x0) - 1)] : FC::one();
+#pragma unroll
+  for (int t = 0; t < NA; ++t) {
+    d[t] = dsum[t].get();
+    if (EQ && fac) d[t] = FC::mul(d[t], wrc);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      c[t * NA + i] = csum[t][i].get();
+      if (EQ && fac) c[t * NA + i] = FC::mul(c[t * NA + i], wrc);
+    }
+  }
+  block_reduce<FC, NA, PASS_THREADS>(d, sh.lds, sh.out);
+  slot_row_add<FC, NA>(a.slots, blk, sh.out);
+  block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
+  la_slot_add<FC, NC>(la_rows(a.slots, 1), blk, NA * W, sh.out);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Round 2 of one proof from the input as 1 + nblk blocks: pair j's tables
+// entries T2[j + h N/8] (h = 0, 1) are sum_m W_m x_m over the input entries
+// x at j + h N/8 + {0, N/4, N/2, 3N/4} with W = ((1-r1)(1-r0), (1-r1) r0,
+// r1 (1-r0), r1 r0) in that order of offsets {0, N/2, N/4, 3N/4}; they are
+// written to `work` and evaluated; block 0 finishes round 2 as in
+// pass_kernel. r_0 is read from the raw point (the control block holds r_1).
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
+    StageArgs<FT, FC> a, uint32_t need_claim) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  __shared__ PassLds<FC, NA> ps;
+  RoundLds<FC, NA>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1, n = a.n;
+  Ctl<FC>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, 2);
+      fin_load(sh.sm, ctl);
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    if (tid == 0) trace_mark(a.trace, 2, 2);
+    slot_rows_sum<FC, NA>(a.slots, sh);
+    if (tid < 64) {
+      finish_round<FT, FC, K, EQ, HOP_PASS>(a, 2, need_claim != 0, sh.out, sh.sm, a.work, a.in,
+                                            a.work_stride, false);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  const size_t q8 = (size_t)1 << (n - 3);  // round 2's pairs
+  const size_t start = (size_t)blk * PASS_THREADS + tid, stride = (size_t)nblk * PASS_THREADS;
+  // Weights of the two folds.
+  T Wt[4];
+  {
+    const T r0 = el_from_words<FC>(reinterpret_cast<const uint32_t*>(a.point));
+    const T r1 = ctl->r;
+    const T u0 = FC::sub(FC::one(), r0), u1 = FC::sub(FC::one(), r1);
+    Wt[0] = FC::mul(u1, u0);  // offset 0
+    Wt[1] = FC::mul(u1, r0);  // offset N/2
+    Wt[2] = FC::mul(r1, u0);  // offset N/4
+    Wt[3] = FC::mul(r1, r0);  // offset 3N/4
+  }
+  const T* wl = nullptr;
+  const T* wr = nullptr;
+  uint32_t nx = 0;
+  bool fac = false;
+  if constexpr (EQ) {
+    select_eq<FC>(2, n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl, wr, nx);
+    fac = wl && (stride & (((size_t)1 << nx) - 1)) == 0;
+  }
+  T acc[NA];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
+  // Pair j: input entries j + m N/8, m = 0..7, of every table.
+  auto load = [&](size_t j, S (&x)[K][8]) {
+    j = j < q8 ? j : q8 - 1;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const S* s = a.in + (size_t)q * a.in_stride;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[q][m] = s[j + (size_t)m * q8];
+    }
+  };
+  S cur[K][8];
+  if (start < q8) load(start, cur);
+  for (size_t j = start; j < q8; j += stride) {
+    T w = FC::one();
+    if constexpr (EQ) w = eq_w<FC>(wl, wr, nx, fac, j);
+    S nxt[K][8];
+    load(j + stride, nxt);
+    T lo[K], df[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      T* d = a.work + (size_t)q * a.work_stride;
+      T v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // Offsets 0, N/2, N/4, 3N/4 of entry j + h N/8: m = h, h + 4, h + 2, h + 6.
+        const S x4[4] = {cur[q][h], cur[q][h + 4], cur[q][h + 2], cur[q][h + 6]};
+        v[h] = mix4<FC, FT>(Wt, x4);
+        d[j + (size_t)h * q8] = v[h];
+      }
+      lo[q] = v[0];
+      df[q] = FC::sub(v[1], v[0]);
+    }
+    if constexpr (EQ) {
+      const T hi0 = FC::add(lo[0], df[0]);
+      lo[0] = FC::mul(w, lo[0]);
+      df[0] = FC::sub(FC::mul(w, hi0), lo[0]);
+    }
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      T prod = lo[0];
+#pragma unroll
+      for (int q = 1; q < K; ++q) prod = FC::mul(prod, lo[q]);
+      acc[t] = FC::add(acc[t], prod);
+      if (t < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) lo[q] = FC::add(lo[q], df[q]);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) cur[q][m] = nxt[q][m];
+  }
+  if (EQ && fac && start < q8) {
+    const T wrc = wr[start & (((size_t)1 << nx) - 1)];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) acc[t] = FC::mul(acc[t], wrc);
+  }
+  block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
+  slot_row_add<FC, NA>(a.slots, blk, sh.out);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------ batched steps
 // One step over many proofs: each item is one round of one proof. Item i owns
 // blocks [block_off, block_off + blocks) of the launch; a tail item is one

@@ -81,7 +81,8 @@ struct KernelRecord {
 };
 enum {
   KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5,
-  KREC_PASS_LA = 6, KREC_CHAIN_LA = 7, KREC_TAIL_LA = 8  // look-ahead variants
+  KREC_PASS_LA = 6, KREC_CHAIN_LA = 7, KREC_TAIL_LA = 8,  // look-ahead variants
+  KREC_FUSED01 = 9, KREC_FOLD2 = 10                       // rounds 0-2 from the input
 };
 
 }  // namespace zs
