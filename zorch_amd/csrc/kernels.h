@@ -2153,6 +2153,128 @@ __device__ __forceinline__ typename FC::T eq_w(const typename FC::T* wl, const t
   return wr[j];
 }
 
+// fused01_kernel's streaming loop: groups g = start + i stride < N/4, one a
+// step (the nine lazy coefficient sums fill the registers), the next step's entries (and, with FAC, its eq weights: wl alone, the
+// per-thread wr factor applied by the caller) loaded before this step's
+// arithmetic. Group g: input entries g + m N/4 (m = 0..3) of every table,
+// i.e. round 0's pairs g and g + N/4 and round 1's pair g.
+template <class FT, class FC, int K, bool EQ, bool FAC>
+__device__ __forceinline__ void fused01_stream(
+    const StageArgs<FT, FC>& a, size_t start, size_t stride, const typename FC::T* wl0,
+    const typename FC::T* wr0, uint32_t x0, const typename FC::T* wl1,
+    const typename FC::T* wr1, uint32_t x1, WeightedSum<FC, FT> (&dsum)[K + 1],
+    WeightedSum<FC, FT> (&csum)[K + 1][K + 1]) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1;
+  constexpr int U = 1;
+  constexpr bool PW = EQ && FAC;  // weights prefetched with the entries
+  constexpr int UW = PW ? U : 1;
+  const size_t q4 = (size_t)1 << (a.n - 2);
+  const size_t step = (size_t)U * stride;
+  auto load = [&](size_t g0, S (&x)[U][K][4], T (&wa)[UW], T (&wb)[UW], T (&wc)[UW]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      size_t g = g0 + (size_t)u * stride;
+      g = g < q4 ? g : q4 - 1;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const S* s = a.in + (size_t)q * a.in_stride;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) x[u][q][m] = s[g + (size_t)m * q4];
+      }
+      if constexpr (PW) {
+        wa[u] = wl0[g >> x0];
+        wb[u] = wl0[(g + q4) >> x0];
+        wc[u] = wl1[g >> x1];
+      }
+    }
+  };
+  S cur[U][K][4];
+  T ca[UW], cb[UW], cc[UW];
+  if (start < q4) load(start, cur, ca, cb, cc);
+  uint32_t par = 0;
+  for (size_t g0 = start; g0 < q4; g0 += step) {
+    S nxt[U][K][4];
+    T na[UW], nb[UW], nc[UW];
+    load(g0 + step, nxt, na, nb, nc);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t g = g0 + (size_t)u * stride;
+      if (g < q4) {
+        T w0a = FC::one(), w0b = FC::one(), w1 = FC::one();
+        if constexpr (PW) {
+          w0a = ca[u];
+          w0b = cb[u];
+          w1 = cc[u];
+        } else if constexpr (EQ) {
+          w0a = eq_w<FC>(wl0, wr0, x0, false, g);
+          w0b = eq_w<FC>(wl0, wr0, x0, false, g + q4);
+          w1 = eq_w<FC>(wl1, wr1, x1, false, g);
+        }
+        const bool fold = (par++ & 1u) != 0;
+        // Round 0: pairs (x0, x2) and (x1, x3).
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          S lo[K], df[K];
+#pragma unroll
+          for (int q = 0; q < K; ++q) {
+            lo[q] = cur[u][q][h];
+            df[q] = FT::sub(cur[u][q][h + 2], cur[u][q][h]);
+          }
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+            S prod = lo[0];
+#pragma unroll
+            for (int q = 1; q < K; ++q) prod = FT::mul(prod, lo[q]);
+            dsum[t].add(h ? w0b : w0a, prod, h == 1);
+            if (t < K) {
+#pragma unroll
+              for (int q = 0; q < K; ++q) lo[q] = FT::add(lo[q], df[q]);
+            }
+          }
+        }
+        // Round 1's pair g: lo = x0 + r (x2 - x0), hi = x1 + r (x3 - x1).
+        S A[K], B[K], dA[K], dB[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          A[q] = cur[u][q][0];
+          B[q] = FT::sub(cur[u][q][2], cur[u][q][0]);
+          dA[q] = FT::sub(cur[u][q][1], cur[u][q][0]);
+          dB[q] = FT::sub(FT::sub(cur[u][q][3], cur[u][q][1]), B[q]);
+        }
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+          S p[K + 1];
+          la_poly<FT, K>(A, B, p);
+#pragma unroll
+          for (int i = 0; i < NA; ++i) csum[t][i].add(w1, p[i], fold);
+          if (t < K) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+              A[q] = FT::add(A[q], dA[q]);
+              B[q] = FT::add(B[q], dB[q]);
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) cur[u][q][m] = nxt[u][q][m];
+      if constexpr (PW) {
+        ca[u] = na[u];
+        cb[u] = nb[u];
+        cc[u] = nc[u];
+      }
+    }
+  }
+}
+
 template <class FC, int K>
 struct Fused01Lds {
   static constexpr int NA = K + 1, NC = NA * NA;
@@ -2241,80 +2363,10 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
 #pragma unroll
     for (int i = 0; i < NA; ++i) csum[t][i].init();
   }
-  // Group g: input entries g + m N/4 (m = 0..3) of every table: round 0's
-  // pairs g and g + N/4, round 1's pair g.
-  auto load = [&](size_t g, S (&x)[K][4]) {
-    g = g < q4 ? g : q4 - 1;
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      const S* s = a.in + (size_t)q * a.in_stride;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) x[q][m] = s[g + (size_t)m * q4];
-    }
-  };
-  S cur[K][4];
-  if (start < q4) load(start, cur);
-  uint32_t par = 0;
-  for (size_t g = start; g < q4; g += stride) {
-    T w0a = FC::one(), w0b = FC::one(), w1 = FC::one();
-    if constexpr (EQ) {
-      w0a = eq_w<FC>(wl0, wr0, x0, fac, g);
-      w0b = eq_w<FC>(wl0, wr0, x0, fac, g + q4);
-      w1 = eq_w<FC>(wl1, wr1, x1, fac, g);
-    }
-    S nxt[K][4];
-    load(g + stride, nxt);
-    const bool fold = (par++ & 1u) != 0;
-    // Round 0: pairs (x0, x2) and (x1, x3).
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      S lo[K], df[K];
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        lo[q] = cur[q][h];
-        df[q] = FT::sub(cur[q][h + 2], cur[q][h]);
-      }
-#pragma unroll
-      for (int t = 0; t < NA; ++t) {
-        S prod = lo[0];
-#pragma unroll
-        for (int q = 1; q < K; ++q) prod = FT::mul(prod, lo[q]);
-        dsum[t].add(h ? w0b : w0a, prod, h == 1);
-        if (t < K) {
-#pragma unroll
-          for (int q = 0; q < K; ++q) lo[q] = FT::add(lo[q], df[q]);
-        }
-      }
-    }
-    // Round 1's pair g: lo = x0 + r (x2 - x0), hi = x1 + r (x3 - x1).
-    S A[K], B[K], dA[K], dB[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      A[q] = cur[q][0];
-      B[q] = FT::sub(cur[q][2], cur[q][0]);
-      dA[q] = FT::sub(cur[q][1], cur[q][0]);
-      dB[q] = FT::sub(FT::sub(cur[q][3], cur[q][1]), B[q]);
-    }
-#pragma unroll
-    for (int t = 0; t < NA; ++t) {
-      S p[K + 1];
-      la_poly<FT, K>(A, B, p);
-#pragma unroll
-      for (int i = 0; i < NA; ++i) csum[t][i].add(w1, p[i], fold);
-      if (t < K) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-          A[q] = FT::add(A[q], dA[q]);
-          B[q] = FT::add(B[q], dB[q]);
-        }
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < K; ++q)
-#pragma unroll
-      for (int m = 0; m < 4; ++m) cur[q][m] = nxt[q][m];
-  }
+  if (fac)
+    fused01_stream<FT, FC, K, EQ, true>(a, start, stride, wl0, wr0, x0, wl1, wr1, x1, dsum, csum);
+  else
+    fused01_stream<FT, FC, K, EQ, false>(a, start, stride, wl0, wr0, x0, wl1, wr1, x1, dsum, csum);
   T d[NA], c[NC];
   const T wrc = EQ && fac && start < q4 ? wr0[start & (((size_t)1 << x0) - 1)] : FC::one();
 #pragma unroll
@@ -2402,8 +2454,9 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
   T acc[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
-  // Pair j: input entries j + m N/8, m = 0..7, of every table.
-  auto load = [&](size_t j, S (&x)[K][8]) {
+  // Pair j: input entries j + m N/8, m = 0..7, of every table, and with
+  // `fac` its eq weight (wl alone; wr is this thread's factor).
+  auto load = [&](size_t j, S (&x)[K][8], T& wv) {
     j = j < q8 ? j : q8 - 1;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -2411,14 +2464,17 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
 #pragma unroll
       for (int m = 0; m < 8; ++m) x[q][m] = s[j + (size_t)m * q8];
     }
+    if (EQ && fac) wv = wl[j >> nx];
   };
   S cur[K][8];
-  if (start < q8) load(start, cur);
+  T cw = FC::one();
+  if (start < q8) load(start, cur, cw);
   for (size_t j = start; j < q8; j += stride) {
-    T w = FC::one();
-    if constexpr (EQ) w = eq_w<FC>(wl, wr, nx, fac, j);
+    T w = cw;
+    if (EQ && !fac) w = eq_w<FC>(wl, wr, nx, false, j);
     S nxt[K][8];
-    load(j + stride, nxt);
+    T nw = FC::one();
+    load(j + stride, nxt, nw);
     T lo[K], df[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -2455,6 +2511,7 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
     for (int q = 0; q < K; ++q)
 #pragma unroll
       for (int m = 0; m < 8; ++m) cur[q][m] = nxt[q][m];
+    cw = nw;
   }
   if (EQ && fac && start < q8) {
     const T wrc = wr[start & (((size_t)1 << nx) - 1)];
