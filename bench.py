@@ -509,6 +509,7 @@ def main() -> int:
     wt = inst.tables.field.itemsize
     wc = inst.chal.itemsize
     alg = algorithmic_bytes(inst.tables.shape[0], inst.n, wt, wc)
+    kernel_bytes = int(sum(g["bytes_per_launch"] * g["launches_per_proof"] for g in groups))
 
     out = {
         "metric": "sumcheck table-elems/s + HBM GB/s at 2^24; batched/served p50/p99 proof latency",
@@ -530,6 +531,11 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "proof_hbm_gbs": alg / (ms_per_step * 1e-3) / 1e9,
         "proof_algorithmic_bytes": alg,
+        # What the launched kernels move per proof (the model above counts a
+        # fold-and-evaluate pass per round; rounds 0-2 here read the input
+        # twice instead of writing and re-reading the first folded tables).
+        "proof_kernel_bytes": kernel_bytes,
+        "proof_kernel_gbs": kernel_bytes / (ms_per_step * 1e-3) / 1e9,
         "verified": bool(ok),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
