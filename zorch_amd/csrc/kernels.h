@@ -2143,6 +2143,44 @@ __device__ __forceinline__ bb4 mix4<BB4, BB>(const bb4 (&W)[4], const uint32_t (
   return bb4_make(o[0], o[1], o[2], o[3]);
 }
 
+// Round 2's entry from its four input entries at offsets 0, N/2, N/4, 3N/4
+// (x), folded at r0 then r1: two folds of the challenge field, or for
+// BabyBear entries under BabyBear^4 one lazy four-term sum with the weights
+// W = ((1-r1)(1-r0), (1-r1) r0, r1 (1-r0), r1 r0).
+template <class FC, class FT>
+struct Fold2 {
+  static __device__ __forceinline__ typename FC::T go(const typename FC::T (&W)[4],
+                                                      typename FC::T r0, typename FC::T r1,
+                                                      const typename FT::T (&x)[4]) {
+    using T = typename FC::T;
+    const T x0 = Lift<FT, FC>::go(x[0]), x1 = Lift<FT, FC>::go(x[1]);
+    const T x2 = Lift<FT, FC>::go(x[2]), x3 = Lift<FT, FC>::go(x[3]);
+    const T a = FC::add(x0, FC::mul(r0, FC::sub(x1, x0)));
+    const T b = FC::add(x2, FC::mul(r0, FC::sub(x3, x2)));
+    return FC::add(a, FC::mul(r1, FC::sub(b, a)));
+  }
+};
+template <>
+struct Fold2<BB4, BB> {
+  static __device__ __forceinline__ bb4 go(const bb4 (&W)[4], bb4, bb4, const uint32_t (&x)[4]) {
+    return mix4<BB4, BB>(W, x);
+  }
+};
+
+// fused01_kernel's sums: eq-weighted (lazily for BabyBear^4 weights on
+// BabyBear values), or without eq plain base-field sums lifted at the end.
+template <class FC, class FT, bool EQ>
+struct FusedSum : WeightedSum<FC, FT> {};
+template <class FC, class FT>
+struct FusedSum<FC, FT, false> {
+  typename FT::T s;
+  __device__ __forceinline__ void init() { s = FT::zero(); }
+  __device__ __forceinline__ void add(typename FC::T, typename FT::T x, bool = true) {
+    s = FT::add(s, x);
+  }
+  __device__ __forceinline__ typename FC::T get() const { return Lift<FT, FC>::go(s); }
+};
+
 // Eq weight of pair j of a round: with `fac`, the per-thread factor wr is
 // applied to the sums at the end and the pair takes wl alone.
 template <class FC>
@@ -2162,8 +2200,8 @@ template <class FT, class FC, int K, bool EQ, bool FAC>
 __device__ __forceinline__ void fused01_stream(
     const StageArgs<FT, FC>& a, size_t start, size_t stride, const typename FC::T* wl0,
     const typename FC::T* wr0, uint32_t x0, const typename FC::T* wl1,
-    const typename FC::T* wr1, uint32_t x1, WeightedSum<FC, FT> (&dsum)[K + 1],
-    WeightedSum<FC, FT> (&csum)[K + 1][K + 1]) {
+    const typename FC::T* wr1, uint32_t x1, FusedSum<FC, FT, EQ> (&dsum)[K + 1],
+    FusedSum<FC, FT, EQ> (&csum)[K + 1][K + 1]) {
   using T = typename FC::T;
   using S = typename FT::T;
   constexpr int NA = K + 1;
@@ -2356,7 +2394,7 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
     const size_t mask = ((size_t)1 << x0) - 1;
     fac = wl0 && wl1 && wr0 == wr1 && x0 == x1 && (stride & mask) == 0 && (q4 & mask) == 0;
   }
-  WeightedSum<FC, FT> dsum[NA], csum[NA][NA];
+  FusedSum<FC, FT, EQ> dsum[NA], csum[NA][NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) {
     dsum[t].init();
@@ -2388,12 +2426,11 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
   if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Round 2 of one proof from the input as 1 + nblk blocks: pair j's tables
-// entries T2[j + h N/8] (h = 0, 1) are sum_m W_m x_m over the input entries
-// x at j + h N/8 + {0, N/4, N/2, 3N/4} with W = ((1-r1)(1-r0), (1-r1) r0,
-// r1 (1-r0), r1 r0) in that order of offsets {0, N/2, N/4, 3N/4}; they are
-// written to `work` and evaluated; block 0 finishes round 2 as in
-// pass_kernel. r_0 is read from the raw point (the control block holds r_1).
+// Round 2 of one proof from the input as 1 + nblk blocks: pair j's table
+// entries T2[j + h N/8] (h = 0, 1) are the input entries at j + h N/8 +
+// {0, N/2, N/4, 3N/4} folded at r_0 and r_1 (Fold2); they are written to
+// `work` and evaluated; block 0 finishes round 2 as in pass_kernel. r_0 is
+// read from the raw point (the control block holds r_1).
 template <class FT, class FC, int K, bool EQ>
 __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
     StageArgs<FT, FC> a, uint32_t need_claim) {
@@ -2434,9 +2471,9 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
   const size_t start = (size_t)blk * PASS_THREADS + tid, stride = (size_t)nblk * PASS_THREADS;
   // Weights of the two folds.
   T Wt[4];
+  const T r0 = el_from_words<FC>(reinterpret_cast<const uint32_t*>(a.point));
+  const T r1 = ctl->r;
   {
-    const T r0 = el_from_words<FC>(reinterpret_cast<const uint32_t*>(a.point));
-    const T r1 = ctl->r;
     const T u0 = FC::sub(FC::one(), r0), u1 = FC::sub(FC::one(), r1);
     Wt[0] = FC::mul(u1, u0);  // offset 0
     Wt[1] = FC::mul(u1, r0);  // offset N/2
@@ -2484,7 +2521,7 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
       for (int h = 0; h < 2; ++h) {
         // Offsets 0, N/2, N/4, 3N/4 of entry j + h N/8: m = h, h + 4, h + 2, h + 6.
         const S x4[4] = {cur[q][h], cur[q][h + 4], cur[q][h + 2], cur[q][h + 6]};
-        v[h] = mix4<FC, FT>(Wt, x4);
+        v[h] = Fold2<FC, FT>::go(Wt, r0, r1, x4);
         d[j + (size_t)h * q8] = v[h];
       }
       lo[q] = v[0];
