@@ -260,7 +260,9 @@ int zs_verify_round(uint32_t chal_field, zs_transcript* transcript, const void* 
  * one batched launch per request kind, so that requests which arrive while
  * others are mid-proof join at the next step. A request whose remaining
  * rounds are small finishes them within one step (at most step_rounds of
- * them per step when that launch shape is set). Results are those of
+ * them per step when that launch shape is set). Requests of 20 rounds or
+ * more are instead proved whole, as by zs_prove, on one of three streams of
+ * the server's own, concurrently with the steps and with one another. Results are those of
  * zs_prove for the same arguments and transcript. Submit, poll, wait and
  * completed may be called from any thread. */
 typedef struct zs_server zs_server;

@@ -2,7 +2,8 @@
 
 Requests submitted at different times (so that later ones join while earlier
 ones are mid-proof) must each equal the proof of the same request made alone,
-bit for bit; admission respects the HBM budget; a request larger than the
+bit for bit, whether they run in the batched steps or, when large, whole on
+the server's own streams; admission respects the HBM budget; a request larger than the
 whole budget fails with MemoryError instead of blocking the queue.
 """
 
@@ -55,6 +56,12 @@ def _requests():
     insts.append(W.product_bb(12, (1404, 1405, 1406), 1407))
     insts.append(W.product_gl(14, (1411, 1412), 1413))
     b3 = W.zerocheck_bb4(12, (1421, 1422), 1423, 1424)
+    b3.transcript_kind = "blake3"
+    insts.append(b3)
+    # Large requests, proved whole on the server's own streams.
+    insts.append(W.zerocheck_bb4(20, (1431, 1432), 1433, 1434))
+    insts.append(W.product_gl(20, (1441, 1442), 1443))
+    b3 = W.zerocheck_bb4(21, (1451, 1452), 1453, 1454)
     b3.transcript_kind = "blake3"
     insts.append(b3)
     return insts

@@ -82,14 +82,16 @@ class InstanceImpl final : public Instance {
  public:
   ~InstanceImpl() override { free_buffers(); }
 
-  int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t) override {
+  using Instance::setup;
+  int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t, hipStream_t s,
+            const TuneParams& tune) override {
     ctx_ = ctx;
     n_ = a.rounds;
     rounds = n_;
     npts = NPTS;
     k = K;
     chal_bytes = sizeof(TC);
-    tune_ = ctx->tune;
+    tune_ = tune;
     const size_t N = (size_t)1 << n_;
     const uint32_t l_half = n_ / 2;
     size_t off = 0;
@@ -139,7 +141,6 @@ class InstanceImpl final : public Instance {
     args_.eq_l_fam = EQ ? (const TC*)(b + o_eql) : nullptr;
     args_.eq_r_fam = EQ ? (const TC*)(b + o_eqr) : nullptr;
     args_.trace = ctx->trace;
-    hipStream_t s = ctx->stream;
     if (stage_in) {
       ZS_HIP(hipMemcpyAsync((void*)args_.in, a.tables, (size_t)K * N * sizeof(TT),
                             hipMemcpyHostToDevice, s));

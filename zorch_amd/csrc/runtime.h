@@ -115,8 +115,13 @@ namespace zs {
 class Instance {
  public:
   virtual ~Instance() = default;
-  // Allocate, upload inputs and the transcript, build eq tables.
-  virtual int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t) = 0;
+  // Allocate, upload inputs and the transcript, build eq tables (on
+  // `stream`, with launch shapes `tune` for every later launch).
+  virtual int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t,
+                    hipStream_t stream, const TuneParams& tune) = 0;
+  int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t) {
+    return setup(ctx, a, t, ctx->stream, ctx->tune);
+  }
   // Enqueue rounds [first, last) on `stream`.
   virtual int enqueue(uint32_t first, uint32_t last, hipStream_t stream) = 0;
   // Enqueue the copy of the outputs to pinned host staging (whole proof).

@@ -41,6 +41,7 @@ struct Pending {
   Clock::time_point t_submit;
   ServeKind* kind = nullptr;
   uint64_t bytes = 0;
+  uint32_t npts = 0;
 };
 
 struct Finished {
@@ -66,6 +67,23 @@ struct Flight {
 };
 constexpr size_t STEPS_IN_FLIGHT = 3;
 
+// A large request proved whole on one of the server's own streams (lanes)
+// by the single-proof kernels, so that its streaming rounds overlap the
+// latency-bound rounds of the others; chained rounds are off there (their
+// blocks wait for one another, which concurrent launches could starve).
+struct Direct {
+  uint64_t id = 0;
+  std::unique_ptr<Instance> inst;
+  hipEvent_t ev = nullptr;
+  uint64_t bytes = 0;
+  Clock::time_point t_submit;
+  uint32_t n = 0, npts = 0, k = 0, cb = 0;
+  bool eq = false;
+};
+constexpr uint32_t DIRECT_MIN_ROUNDS = 20;
+constexpr int LANES = 3;
+constexpr size_t TR_BYTES = sizeof(zs_transcript) - offsetof(zs_transcript, sha);
+
 }  // namespace zs
 
 using namespace zs;
@@ -90,6 +108,9 @@ struct zs_server {
   std::deque<Flight> flights;
   std::vector<hipEvent_t> free_events;
   Clock::time_point busy_until{};  // end of the busy time already counted
+  std::vector<hipStream_t> lanes;
+  size_t lane_next = 0;
+  std::vector<Direct> directs;
 };
 
 namespace {
@@ -103,7 +124,7 @@ void publish(zs_server* S, std::vector<std::pair<uint64_t, Finished>>& done, uin
     S->completed_ids.push_back(d.first);
     S->finished[d.first] = std::move(d.second);
   }
-  S->stats.active = S->active.size();
+  S->stats.active = S->active.size() + S->directs.size();
   S->stats.resident_bytes = S->resident;
   done.clear();
   S->cv_done.notify_all();
@@ -113,6 +134,7 @@ void publish(zs_server* S, std::vector<std::pair<uint64_t, Finished>>& done, uin
 // in-flight request.
 void fail_all_active(zs_server* S, int rc, const std::string& err) {
   (void)hipStreamSynchronize(S->ctx->stream);
+  for (hipStream_t l : S->lanes) (void)hipStreamSynchronize(l);
   std::vector<std::pair<uint64_t, Finished>> done;
   uint64_t freed = 0;
   auto fail_one = [&](Active& a) {
@@ -130,7 +152,108 @@ void fail_all_active(zs_server* S, int rc, const std::string& err) {
     S->free_events.push_back(fl.ev);
   }
   S->flights.clear();
+  for (Direct& d : S->directs) {
+    freed += d.bytes;
+    Finished f;
+    f.rc = rc;
+    f.err = err;
+    done.emplace_back(d.id, std::move(f));
+    if (d.ev) S->free_events.push_back(d.ev);
+  }
+  S->directs.clear();
   publish(S, done, freed);
+}
+
+// Start request p whole on the next lane; a failure is reported in `done`.
+void direct_start(zs_server* S, Pending& p, std::vector<std::pair<uint64_t, Finished>>& done,
+                  uint64_t& freed) {
+  zs_ctx* ctx = S->ctx;
+  const bool eq = p.a.mode == ZS_MODE_EQ;
+  Direct d;
+  d.id = p.id;
+  d.bytes = p.bytes;
+  d.t_submit = p.t_submit;
+  d.n = p.a.rounds;
+  d.npts = p.npts;
+  d.k = p.a.num_tables;
+  d.cb = field_bytes(p.a.chal_field);
+  d.eq = eq;
+  d.inst.reset(make_instance(p.a.table_field, p.a.chal_field, p.a.num_tables, p.npts, eq));
+  int rc = d.inst ? ZS_OK : fail(ZS_EINVAL, "unsupported field / degree combination");
+  const hipStream_t lane = S->lanes[S->lane_next++ % S->lanes.size()];
+  TuneParams tune = ctx->tune;
+  tune.chain_pairs = 0;
+  if (!rc) rc = d.inst->setup(ctx, p.a, p.t, lane, tune);
+  if (!rc) rc = d.inst->enqueue(0, d.n, lane);
+  if (!rc) rc = d.inst->download(lane);
+  if (!rc) {
+    if (S->free_events.empty()) {
+      hipEvent_t e = nullptr;
+      const hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if (he != hipSuccess) rc = hip_fail(he, "hipEventCreateWithFlags");
+      else S->free_events.push_back(e);
+    }
+  }
+  if (!rc) {
+    d.ev = S->free_events.back();
+    S->free_events.pop_back();
+    const hipError_t he = hipEventRecord(d.ev, lane);
+    if (he != hipSuccess) {
+      S->free_events.push_back(d.ev);
+      d.ev = nullptr;
+      rc = hip_fail(he, "hipEventRecord");
+    }
+  }
+  if (rc) {
+    if (d.inst) {
+      (void)hipStreamSynchronize(lane);
+      d.inst->free_buffers();
+    }
+    Finished f;
+    f.rc = rc;
+    f.err = zs_last_error();
+    freed += p.bytes;
+    done.emplace_back(p.id, std::move(f));
+    return;
+  }
+  S->directs.push_back(std::move(d));
+}
+
+// Publish every lane proof whose event has completed.
+void collect_directs(zs_server* S) {
+  std::vector<std::pair<uint64_t, Finished>> done;
+  uint64_t freed = 0;
+  std::vector<Direct> still;
+  for (Direct& d : S->directs) {
+    const hipError_t q = hipEventQuery(d.ev);
+    if (q == hipErrorNotReady) {
+      still.push_back(std::move(d));
+      continue;
+    }
+    Finished f;
+    if (q != hipSuccess) {
+      f.rc = hip_fail(q, "hipEventQuery");
+      f.err = zs_last_error();
+    } else {
+      ServeResult& r = f.res;
+      r.msgs.resize((size_t)d.n * d.npts * d.cb);
+      r.point.resize((size_t)d.n * d.cb);
+      r.value.resize(d.cb);
+      r.final_tables.resize((size_t)(d.k + (d.eq ? 1 : 0)) * d.cb);
+      zs_proof pr{r.msgs.data(), r.point.data(), r.value.data(), r.final_tables.data()};
+      zs_transcript tr{};
+      f.rc = d.inst->collect(&pr, &tr);
+      if (f.rc) f.err = zs_last_error();
+      r.tr_state.assign((const uint8_t*)&tr.sha, (const uint8_t*)&tr.sha + TR_BYTES);
+      f.latency = std::chrono::duration<double>(Clock::now() - d.t_submit).count();
+    }
+    d.inst->free_buffers();
+    S->free_events.push_back(d.ev);
+    freed += d.bytes;
+    done.emplace_back(d.id, std::move(f));
+  }
+  S->directs.swap(still);
+  if (!done.empty() || freed) publish(S, done, freed);
 }
 
 // Hand the results of the oldest in-flight step to pollers (its event has
@@ -163,19 +286,28 @@ void collect_flight(zs_server* S) {
 void server_loop(zs_server* S) {
   zs_ctx* ctx = S->ctx;
   (void)hipSetDevice(ctx->device);
+  for (int i = 0; i < LANES; ++i) {
+    hipStream_t l = nullptr;
+    if (hipStreamCreateWithFlags(&l, hipStreamNonBlocking) != hipSuccess) break;
+    S->lanes.push_back(l);
+  }
   std::vector<Pending> admit;
   std::vector<std::pair<uint64_t, Finished>> done;
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (S->active.empty() && S->flights.empty())
-        S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+      auto ready = [&] { return S->stop || !S->queue.empty(); };
+      if (S->active.empty() && S->flights.empty()) {
+        if (S->directs.empty()) S->cv_work.wait(lk, ready);
+        else S->cv_work.wait_for(lk, std::chrono::microseconds(50), ready);
+      }
       if (S->stop) break;
       // Arrival order; a request that can never fit fails instead of blocking.
       while (!S->queue.empty()) {
         Pending& p = S->queue.front();
         if (S->resident + p.bytes > S->budget) {
-          if (!S->active.empty() || !S->flights.empty() || !admit.empty()) break;
+          if (!S->active.empty() || !S->flights.empty() || !S->directs.empty() || !admit.empty())
+            break;
           Finished f;
           f.rc = ZS_EBUDGET;
           f.err = "request needs " + std::to_string(p.bytes) + " device bytes; server budget is " +
@@ -199,6 +331,10 @@ void server_loop(zs_server* S) {
     for (Pending& p : admit) {
       p.a.claim = p.claim.data();
       p.a.eq_point = p.eq_point.empty() ? nullptr : p.eq_point.data();
+      if (p.a.rounds >= DIRECT_MIN_ROUNDS && !S->lanes.empty()) {
+        direct_start(S, p, done, freed);
+        continue;
+      }
       ServeReq* r = nullptr;
       const int rc = p.kind->admit(ctx, p.a, p.t, &r);
       if (rc) {
@@ -219,7 +355,8 @@ void server_loop(zs_server* S) {
     {
       std::lock_guard<std::mutex> lk(S->mu);
       S->stats.admitted += admit.size();
-      if (S->active.size() > S->stats.peak_active) S->stats.peak_active = S->active.size();
+      const size_t live = S->active.size() + S->directs.size();
+      if (live > S->stats.peak_active) S->stats.peak_active = live;
     }
     admit.clear();
     if (!done.empty() || freed) publish(S, done, freed);
@@ -267,7 +404,7 @@ void server_loop(zs_server* S) {
       {
         std::lock_guard<std::mutex> lk(S->mu);
         ++S->stats.steps;
-        S->stats.active = S->active.size();
+        S->stats.active = S->active.size() + S->directs.size();
       }
     } else if (!S->flights.empty()) {
       // Nothing to launch: wait for the oldest step.
@@ -277,7 +414,7 @@ void server_loop(zs_server* S) {
         continue;
       }
     }
-    // Results of every completed step, oldest first.
+    // Results of every completed step, oldest first, and of finished lanes.
     while (!S->flights.empty()) {
       const hipError_t q = hipEventQuery(S->flights.front().ev);
       if (q == hipErrorNotReady) break;
@@ -287,6 +424,7 @@ void server_loop(zs_server* S) {
       }
       collect_flight(S);
     }
+    if (!S->directs.empty()) collect_directs(S);
   }
   // Stopping: let queued steps finish, then drop every request.
   (void)hipStreamSynchronize(ctx->stream);
@@ -297,6 +435,14 @@ void server_loop(zs_server* S) {
   S->flights.clear();
   for (Active& a : S->active) a.kind->release(ctx, a.req.get());
   S->active.clear();
+  for (hipStream_t l : S->lanes) (void)hipStreamSynchronize(l);
+  for (Direct& d : S->directs) {
+    d.inst->free_buffers();
+    S->free_events.push_back(d.ev);
+  }
+  S->directs.clear();
+  for (hipStream_t l : S->lanes) (void)hipStreamDestroy(l);
+  S->lanes.clear();
   for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
   S->free_events.clear();
 }
@@ -368,6 +514,7 @@ int zs_server_submit(zs_server* S, const zs_prove_args* args, const zs_transcrip
   }
   p.kind = kind.get();
   p.bytes = kind->bytes_needed(*args);
+  p.npts = npts;
   p.id = S->next_id++;
   *id = p.id;
   S->known.insert(p.id);
