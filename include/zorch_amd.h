@@ -170,9 +170,10 @@ int zs_ctx_trace_read(zs_ctx* ctx, uint64_t* pairs, size_t cap, size_t* count,
  * chain_grid blocks (power of two; chain_pairs 0: one launch per round).
  * In batched and served proving a proof's tail runs at most step_rounds rounds
  * per step (0: all of its remaining rounds in one step). lookahead (0/1): in
- * zs_prove, the single-block tail (when it starts at the third round or later
- * and runs at least two rounds) folds and evaluates each round while the
- * previous round's message is hashed.
+ * zs_prove, rounds 0-2 come from the input in two launches (round 1's tables
+ * are never written) when the stage starts with three pass rounds, and the
+ * tail, the chained launch and the pass round before it fold and evaluate
+ * each round while the previous round's message is hashed.
  * Results do not depend on them. Set applies to proofs started afterwards. */
 typedef struct zs_tune {
   uint32_t tail_pairs;

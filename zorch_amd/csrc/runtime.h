@@ -70,7 +70,7 @@ struct TuneParams {
                                     // tail_pairs) run chained in one persistent launch; 0: off
   uint32_t chain_grid = 128;        // blocks of a chained launch (at most)
   uint32_t step_rounds = 0;         // batched / served: tail rounds per step (0: all)
-  uint32_t lookahead = 1;           // zs_prove: tail rounds from the third on overlap the hash
+  uint32_t lookahead = 1;           // zs_prove: rounds 0-2 from the input; look-ahead rounds
 };
 
 // Kernel timing records (enabled by zs_ctx_profile).
