@@ -240,7 +240,7 @@ class InstanceImpl final : public Instance {
         hipEvent_t ev = begin(s);
         const bool la = tune_.lookahead && r >= 2 && last == n_ && n_ - r >= 2;
         if (la)
-          tail_la_kernel<FT, FC, K, EQ><<<1, LA_THREADS, 0, s>>>(args_, r);
+          tail_la_kernel<FT, FC, K, EQ><<<1, LA_THREADS, 0, s>>>(args_, r, cont ? 1u : 0u);
         else
           tail_kernel<FT, FC, K, EQ><<<1, TAIL_THREADS, 0, s>>>(args_, r, last);
         ZS_HIP(hipGetLastError());
@@ -258,7 +258,9 @@ class InstanceImpl final : public Instance {
         const bool la = tune_.lookahead && end - r >= 2;
         int rc = la ? enqueue_chain_la(r, end, last, cont, s) : enqueue_chain(r, end, last, s);
         if (rc) return rc;
-        cont = false;
+        // A look-ahead chain that stops before the last round leaves the next
+        // round's coefficients for the tail.
+        cont = la && end < n_;
         r = end - 1;
         continue;
       }

@@ -1428,6 +1428,17 @@ __global__ __launch_bounds__(TAIL_THREADS) void tail_kernel(StageArgs<FS, FC> a,
   tail_rounds<FS, FC, K, EQ, TAIL_THREADS>(a, first, last, sh);
 }
 
+// Slot rows of the look-ahead coefficient sums.
+constexpr int LA_NSLOT = 8;     // slot rows per parity
+constexpr int LA_SLOTW = 128;   // 64-bit words per row: round sums, then coefficients
+constexpr size_t LA_SLOT_WORDS = (size_t)2 * LA_NSLOT * LA_SLOTW;
+
+// Look-ahead rows of the sums of round x (parity x & 1): after the pass and
+// chained kernels' slot rows.
+__device__ __forceinline__ uint64_t* la_rows(uint64_t* slots, uint32_t x) {
+  return slots + (size_t)NSLOT * SLOTW + (size_t)(x & 1u) * LA_NSLOT * LA_SLOTW;
+}
+
 // ---------------------------------------------------- look-ahead tail rounds
 // The tail rounds [first, n) (first >= 2, at least two rounds) of one proof
 // in one block, with each round's folding and evaluation run while the
@@ -1469,6 +1480,7 @@ struct TailLaLds {
   FinishLds<FC> sm;
   typename FC::T part[2][LA_FIRST_WAVES][NP];
   typename FC::T coef[NA * NA];
+  uint64_t gsum[NA * NA * 4];  // round first's coefficient words from the slot rows (cont)
   typename FC::T out[NA];
   typename FC::T pub_r;  // r_{k-1} for period k
   uint32_t go;           // periods allowed to start: every k < go
@@ -1671,9 +1683,12 @@ __device__ __forceinline__ void la_arrive(typename FC::T (&v)[NV], bool any,
   if (lane == 0) __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// With `cont`, the previous launch (chain_la_kernel) left round first's
+// coefficients in the look-ahead slot rows: the first message is hashed at
+// once and the first period is an ordinary one.
 template <class FS, class FC, int K, bool EQ>
 __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a,
-                                                             uint32_t first) {
+                                                             uint32_t first, uint32_t cont) {
   using T = typename FC::T;
   constexpr int NA = K + 1, NC = NA * NA;
   __shared__ TailLaLds<FC, K> sh;
@@ -1691,7 +1706,7 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   const int widx = wave == 0 || wave == 4 ? -1 : (int)(wave < 4 ? wave - 1 : wave - 2);
   const uint32_t q8f = 1u << (n - first - 2);  // pairs of round first+1
   // Every wave: the first period's fold and round first's node sums.
-  {
+  if (!cont) {
     ChalMul<FC> cm;
     cm.init(sh.pub_r);
     T d[NA], c[NC];
@@ -1702,11 +1717,11 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   }
   if (widx >= 0) {
     // Round first+1's coefficients, once every wave has folded.
-    if (!la_wait(&sh.arrived, LA_FIRST_WAVES)) {
+    if (!cont && !la_wait(&sh.arrived, LA_FIRST_WAVES)) {
       sh.fault = 1;
       return;
     }
-    {
+    if (!cont) {
       ChalMul<FC> cm;
       T d[NA], c[NC];
 #pragma unroll
@@ -1716,8 +1731,8 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
       la_arrive<FC, NC>(c, (uint32_t)widx * 64u < q8f, sh.part[first & 1][widx] + NA,
                         &sh.arrived);
     }
-    // Later periods.
-    for (uint32_t k = first + 1; k < n; ++k) {
+    // Later periods (with `cont`, from the first).
+    for (uint32_t k = cont ? first : first + 1; k < n; ++k) {
       if (!la_wait(&sh.go, k + 1)) {
         sh.fault = 1;
         return;
@@ -1743,12 +1758,40 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   }
   if (wave != 0) return;
   // The finisher.
-  uint32_t target = LA_FIRST_WAVES;
+  uint32_t target = cont ? 0u : LA_FIRST_WAVES;
   bool ok = true;
   for (uint32_t k = first; k < n && ok; ++k) {
     if (lane == 0) trace_mark(a.trace, k, 1);
     fin_eq<FS, FC, EQ>(sh.sm, a, k);
-    if (k == first) {
+    if (k == first && cont) {
+      // Round first's node sums: the previous launch's coefficients at
+      // r_{first-1} (complete at the launch boundary: plain loads, all rows
+      // at once).
+      constexpr int W = SlotOps<FC>::W;
+      uint64_t* rows = la_rows(a.slots, first) + NA * W;
+      for (uint32_t w = lane; w < (uint32_t)(NC * W); w += 64u) {
+        uint64_t v[LA_NSLOT];
+#pragma unroll
+        for (int row = 0; row < LA_NSLOT; ++row) v[row] = rows[(size_t)row * LA_SLOTW + w];
+        uint64_t s = 0;
+#pragma unroll
+        for (int row = 0; row < LA_NSLOT; ++row) {
+          s += v[row];
+          rows[(size_t)row * LA_SLOTW + w] = 0;
+        }
+        sh.gsum[w] = s;
+      }
+      wave_sync();
+      if (lane < (uint32_t)NC) sh.coef[lane] = SlotOps<FC>::join(&sh.gsum[lane * W]);
+      wave_sync();
+      if (lane < (uint32_t)NA) {
+        const T r = sh.sm.r;
+        T v = sh.coef[lane * NA + K];
+#pragma unroll
+        for (int i = K - 1; i >= 0; --i) v = FC::add(sh.coef[lane * NA + i], FC::mul(r, v));
+        sh.out[lane] = v;
+      }
+    } else if (k == first) {
       ok = la_wait(&sh.arrived, target);
       if (!ok) break;
       // Round first's node sums, from every wave of the first period.
@@ -1818,24 +1861,14 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
 // message, which it forms from C_k and r_{k-1}. The first period folds and
 // sums round `first`'s node values directly, then sums C_{first+1} from the
 // folded tables while that message is hashed. The period of round last-1
-// only folds: the next launch (or, when last = n, block 0's final fold)
-// reads round last-1's tables. Cross-block sums go through two parities of
+// also sums round last's coefficients, for the tail launch that follows
+// (`cont`); when last = n it only folds, for block 0's final fold. Cross-block sums go through two parities of
 // slot rows (the coefficients of period k land in parity k & 1, read and
 // cleared by block 0 before it lets period k+2 start); every block that works
 // in a period arrives at the control block's counter (release), and block 0
 // publishes r_k (`done`, release) once every block has arrived for period k,
 // whose tables period k+1 reads. The claim is reduced at round
 // claim_last - 1 only. Waits are bounded as in chain_kernel.
-constexpr int LA_NSLOT = 8;     // slot rows per parity
-constexpr int LA_SLOTW = 128;   // 64-bit words per row: round sums, then coefficients
-constexpr size_t LA_SLOT_WORDS = (size_t)2 * LA_NSLOT * LA_SLOTW;
-
-// Look-ahead rows of the sums of round x (parity x & 1): after the pass and
-// chained kernels' slot rows.
-__device__ __forceinline__ uint64_t* la_rows(uint64_t* slots, uint32_t x) {
-  return slots + (size_t)NSLOT * SLOTW + (size_t)(x & 1u) * LA_NSLOT * LA_SLOTW;
-}
-
 // Blocks of a period with `units` thread units of work: one per PASS_THREADS,
 // at least one, at most grid.
 __device__ __forceinline__ uint32_t la_blocks(uint32_t units, uint32_t grid) {
@@ -1843,10 +1876,10 @@ __device__ __forceinline__ uint32_t la_blocks(uint32_t units, uint32_t grid) {
   return b < 1u ? 1u : b < grid ? b : grid;
 }
 
-// Thread units of period k of [first, last) (n rounds): round k+1's pairs,
-// or for the fold-only period the fold's groups of four entries.
-__device__ __forceinline__ uint32_t la_units(uint32_t n, uint32_t k, uint32_t last) {
-  if (k + 1 < last) return 1u << (n - k - 2);
+// Thread units of period k (n rounds): round k+1's pairs, or for the
+// fold-only period of round n-1 its entries.
+__device__ __forceinline__ uint32_t la_units(uint32_t n, uint32_t k) {
+  if (k + 1 < n) return 1u << (n - k - 2);
   const uint32_t L = 1u << (n - k);
   return L >= 4u ? L / 4u : L;
 }
@@ -1920,7 +1953,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
   Ctl<FC>* ctl = a.ctl;
   // Blocks of the first period (the busiest): with `cont`, P_first folds and
   // sums coefficients; else its fold-and-sums part, then its coefficients.
-  const uint32_t nb0 = la_blocks(cont ? la_units(n, first, last) : 1u << (n - first - 2), G);
+  const uint32_t nb0 = la_blocks(cont ? la_units(n, first) : 1u << (n - first - 2), G);
   if (blockIdx.x == 0) {
     // The finisher.
     if (tid < 64) fin_load(sh.sm, ctl);
@@ -1956,7 +1989,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
                                                a.work, a.in, a.work_stride, false);
       }
       // Period k complete: publish r_k, which starts period k+1.
-      target += k == first ? nb0 : la_blocks(la_units(n, k, last), G);
+      target += k == first ? nb0 : la_blocks(la_units(n, k), G);
       if (tid == 0) {
         bool ok = chain_wait(&ctl->counter, target, &ctl->fault);
         if (ok && k + 1 < last) {
@@ -2016,7 +2049,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
     arrive();
   }
   for (uint32_t k = cont ? first : first + 1; k < last; ++k) {
-    const uint32_t nb = la_blocks(la_units(n, k, last), G);
+    const uint32_t nb = la_blocks(la_units(n, k), G);
     if (blk >= nb) return;
     if (tid == 0) {
       const bool ok = chain_wait(&ctl->done, k - first, &ctl->fault);
@@ -2029,7 +2062,7 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
     cm.init(sh.r);
     T d[NA], c[NC];
     const uint32_t wt = blk * PASS_THREADS + tid, nt = nb * PASS_THREADS;
-    if (k + 1 < last) {
+    if (k + 1 < n) {
 #pragma unroll
       for (int i = 0; i < NC; ++i) c[i] = FC::zero();
       la_period<FT, FC, K, EQ, LA_FOLD_C>(a, k, wt, nt, cm, d, c);
