@@ -1860,15 +1860,18 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
 // round k+1's node sums (polynomials in r_k) while block 0 hashes round k's
 // message, which it forms from C_k and r_{k-1}. The first period folds and
 // sums round `first`'s node values directly, then sums C_{first+1} from the
-// folded tables while that message is hashed. The period of round last-1
-// also sums round last's coefficients, for the tail launch that follows
-// (`cont`); when last = n it only folds, for block 0's final fold. Cross-block sums go through two parities of
-// slot rows (the coefficients of period k land in parity k & 1, read and
-// cleared by block 0 before it lets period k+2 start); every block that works
+// folded tables while that message is hashed; with `cont` the previous
+// launch left C_first and the first period is an ordinary one. The period
+// of round last-1 also sums round last's coefficients, for the tail launch
+// that follows; when last = n it only folds, for block 0's final fold.
+// Cross-block sums go through two parities of slot rows (the sums of round x
+// land in parity x & 1, and block 0 reads and clears round k's before it
+// lets period k+1, which writes round k+2's, start); every block that works
 // in a period arrives at the control block's counter (release), and block 0
 // publishes r_k (`done`, release) once every block has arrived for period k,
 // whose tables period k+1 reads. The claim is reduced at round
 // claim_last - 1 only. Waits are bounded as in chain_kernel.
+
 // Blocks of a period with `units` thread units of work: one per PASS_THREADS,
 // at least one, at most grid.
 __device__ __forceinline__ uint32_t la_blocks(uint32_t units, uint32_t grid) {
