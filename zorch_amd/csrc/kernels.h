@@ -2710,11 +2710,25 @@ __device__ __forceinline__ void eq_family_entry(const typename FC::T* v, uint32_
   fam[idx] = acc;
 }
 
+// The point's coordinates staged in LDS first: each entry reads up to n of
+// them in turn, and from global memory every read waited a cache latency.
+constexpr uint32_t EQ_STAGE_MAX = 64;
+template <class FC>
+__device__ __forceinline__ const typename FC::T* eq_stage(const typename FC::T* v, uint32_t n,
+                                                          typename FC::T* vs) {
+  if (n > EQ_STAGE_MAX) return v;
+  if (threadIdx.x < n) vs[threadIdx.x] = v[threadIdx.x];
+  __syncthreads();
+  return vs;
+}
+
 template <class FC>
 __global__ __launch_bounds__(256) void eq_family_kernel(const typename FC::T* v, uint32_t h_l,
                                                         typename FC::T* fam_l, uint32_t h_r,
                                                         typename FC::T* fam_r) {
-  eq_family_entry<FC>(v, h_l, fam_l, h_r, fam_r, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  __shared__ typename FC::T vs[EQ_STAGE_MAX];
+  const typename FC::T* w = eq_stage<FC>(v, h_l + h_r, vs);
+  eq_family_entry<FC>(w, h_l, fam_l, h_r, fam_r, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // The eq families of many proofs in one launch: job i owns blocks
@@ -2738,7 +2752,9 @@ __global__ __launch_bounds__(256) void eq_family_batch_kernel(const EqFamJob<FC>
     if (jobs[mid].block_off <= blockIdx.x) lo = mid; else hi = mid;
   }
   const EqFamJob<FC> j = jobs[lo];
-  eq_family_entry<FC>(j.v, j.h_l, j.fam_l, j.h_r, j.fam_r,
+  __shared__ typename FC::T vs[EQ_STAGE_MAX];
+  const typename FC::T* w = eq_stage<FC>(j.v, j.h_l + j.h_r, vs);
+  eq_family_entry<FC>(w, j.h_l, j.fam_l, j.h_r, j.fam_r,
                       (size_t)(blockIdx.x - j.block_off) * 256 + threadIdx.x);
 }
 
