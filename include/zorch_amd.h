@@ -40,6 +40,10 @@ extern "C" {
 #define ZS_TR_SHA256 1 /* Sha256FieldTranscript: counter squeeze SHA256(S || le64(ctr)) */
 #define ZS_TR_BLAKE3 2 /* Blake3FieldTranscript: squeeze = BLAKE3 extendable output of S */
 
+/* Largest number of rounds (variables) a stage may have: every table of a
+ * round holds at most 2^31 entries. */
+#define ZS_MAX_ROUNDS 31
+
 /* Prover modes. */
 #define ZS_MODE_PRODUCT 1 /* StandardRound(ProductSummand(degree)), natural domain */
 #define ZS_MODE_EQ 2      /* EqPolyProver(ProductSummand(degree)), natural domain */
@@ -86,6 +90,31 @@ typedef struct zs_transcript {
   };
 } zs_transcript;
 
+/* Round-message wire forms. The round polynomial s has degree
+ * D = degree (product) or degree + 1 (eq); npts = D + 1. */
+#define ZS_WIRE_VALUES 0     /* s(0..D), the natural domain (StandardRound's and
+                                EqPolyProver's default; reduce_evals) */
+#define ZS_WIRE_DOMAIN 1     /* s sampled at an explicit evaluation domain: the
+                                finite nodes in order, s(inf) (the leading
+                                coefficient) inserted at inf_pos when has_inf
+                                (EvalDomain.sample, zorch/sumcheck/domain.py:95-106;
+                                the compressed U-hat domain, :140-146;
+                                reduce_domain, zorch/sumcheck/reduce.py:43-52) */
+#define ZS_WIRE_COEFFS 2     /* ascending coefficients of s (product_round_coeffs,
+                                zorch/sumcheck/domain.py:216-223; reduce_coeffs,
+                                zorch/sumcheck/reduce.py:35-40) */
+#define ZS_WIRE_COMPRESSED 3 /* [c0, c2] of a two-factor product round
+                                (CompressedProductRound, zorch/sumcheck/prover.py:177-212;
+                                reduce_compressed, zorch/sumcheck/reduce.py:55-66) */
+
+typedef struct zs_wire {
+  uint32_t kind;      /* ZS_WIRE_* */
+  uint32_t num_nodes; /* ZS_WIRE_DOMAIN: finite sample nodes (distinct) */
+  uint32_t has_inf;   /* ZS_WIRE_DOMAIN: 1 if s(inf) is among the samples */
+  uint32_t inf_pos;   /* ZS_WIRE_DOMAIN: its position, 0..num_nodes */
+  const void* nodes;  /* ZS_WIRE_DOMAIN: num_nodes challenge-field elements (host) */
+} zs_wire;
+
 typedef struct zs_prove_args {
   uint32_t table_field;      /* storage field of `tables` */
   uint32_t chal_field;       /* challenge / message / claim field */
@@ -98,11 +127,16 @@ typedef struct zs_prove_args {
   uint32_t reserved;
   const void* eq_point;      /* ZS_MODE_EQ: n challenge-field elements (host) */
   const void* claim;         /* one challenge-field element (host) */
+  const zs_wire* wire;       /* message wire form; null: ZS_WIRE_VALUES (copied) */
+  const void* eq_mass;       /* ZS_MODE_EQ: the eq mass already bound, one challenge-field
+                                element (EqPolyState's second part); null: one */
 } zs_prove_args;
 
-/* Caller-allocated host outputs. npts = degree + 1 (product) or degree + 2 (eq). */
+/* Caller-allocated host outputs. mlen = the message width of the wire form
+ * (zs_prove_check): npts for values and coefficients, the sample count of a
+ * domain, 2 for the compressed form. */
 typedef struct zs_proof {
-  void* msgs;         /* rounds x npts challenge-field elements */
+  void* msgs;         /* rounds x mlen challenge-field elements */
   void* point;        /* rounds x challenge-field raw squeezed words */
   void* value;        /* one challenge-field element */
   void* final_tables; /* optional: k folded values (+ eq mass for ZS_MODE_EQ) */
@@ -207,6 +241,14 @@ int zs_transcript_squeeze(zs_transcript* t, size_t nbytes, uint8_t* out);
  * zorch/sha256_field_transcript.py:277-284, blake3_field_transcript.py:409). */
 int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]);
 
+/* ---- argument check (host) ----------------------------------------------
+ * The structural checks every prove entry point makes, with no device work:
+ * ZS_EINVAL (message in zs_last_error) for a malformed request, else ZS_OK
+ * and the message width on the wire (elements per round). `transcript` may
+ * be null. */
+int zs_prove_check(const zs_prove_args* args, const zs_transcript* transcript,
+                   uint32_t* msg_len);
+
 /* ---- prover ------------------------------------------------------------
  * Whole stage on the device: SumcheckProver.prove / EqPolyProver.prove
  * (zorch/sumcheck/stage.py:52-69, zorch/sumcheck/eq/stage.py:62-92) driving
@@ -250,10 +292,27 @@ int zs_verify_round(uint32_t chal_field, zs_transcript* transcript, const void* 
                     const void* msg, uint32_t npts, uint32_t degree, void* chal_out,
                     void* reduced_out, int* ok);
 
+/* ---- device field arithmetic (self-test) ---------------------------------
+ * No counterpart in the reference, whose field arithmetic lives in its
+ * third-party dtypes (zk_dtypes): runs `count` independent operations of the
+ * device field library on the GPU (one kernel on the context's stream), the
+ * same functions the round kernels call, for known-answer tests of edge
+ * operands. a, b, out: host arrays of `count` storage elements (operands are
+ * reduced first, as the kernels reduce squeezed words). */
+#define ZS_OP_ADD 1
+#define ZS_OP_SUB 2
+#define ZS_OP_MUL 3
+int zs_field_op(zs_ctx* ctx, uint32_t field, uint32_t op, const void* a, const void* b,
+                void* out, size_t count);
+
 /* ---- proof server (continuous batching) ----------------------------------
  * No counterpart in the reference, which proves one stage per call; this is
  * the serving loop for many independent requests on one GPU. A server takes
- * over a context (do not use the context elsewhere until zs_server_destroy).
+ * over a context: its scheduler thread owns the context's stream and launch
+ * shapes, so do not prove on that context elsewhere until zs_server_destroy
+ * (its device and pinned pools are thread-safe, so device buffers from it
+ * may still be allocated and freed; inputs may also live on another context
+ * of the same device).
  * Its scheduler thread admits submitted requests in arrival order against an
  * HBM budget (resident state per request: control block, outputs, eq
  * families, folded tables, and the input tables when they are on the host),
@@ -275,8 +334,8 @@ typedef struct zs_server_stats {
 } zs_server_stats;
 /* hbm_budget_bytes = 0: the context's budget. */
 int zs_server_create(zs_ctx* ctx, uint64_t hbm_budget_bytes, zs_server** out);
-/* args->tables must stay valid until the request completes; the claim and
- * eq point are copied; the transcript is read. A request larger than the
+/* args->tables must stay valid until the request completes; the claim, the
+ * eq point and the wire form (with its nodes) are copied; the transcript is read. A request larger than the
  * whole budget completes with ZS_EBUDGET. */
 int zs_server_submit(zs_server* srv, const zs_prove_args* args, const zs_transcript* transcript,
                      uint64_t* id);

@@ -263,3 +263,192 @@ def eval_mle(F, evals, point):
     for r in point:
         cur = _fold(F, cur, r)
     return cur[0]
+
+
+# --- message wire forms (TEST INFRASTRUCTURE ONLY) ---------------------------
+# A wire is ("values",), ("domain", nodes, inf_index), ("coeffs",) or
+# ("compressed",): the samples or coefficients a round sends, restated
+# directly from the reference's sampling (EvalDomain.sample, domain.py:95-106;
+# summand_evals, domain.py:176-205; sumcheck_poly_from_t, eq_poly.py:100-111),
+# coefficient maps (EvalDomain.to_coeffs, domain.py:72-93;
+# product_round_coeffs, domain.py:216-223; EqPolyRound._round_coeffs,
+# eq_poly.py:189-206; CompressedProductRound._round_poly, prover.py:191-204)
+# and reductions (reduce.py:27-66).
+
+def interp(F, xs, ys):
+    """Ascending coefficients of the interpolant through (xs[i], ys[i]):
+    the inverse Vandermonde applied to the values, solved by Gauss-Jordan
+    elimination on the Vandermonde system (an independent route from the
+    Lagrange expansion of natural_to_coeffs)."""
+    m = len(xs)
+    rows = []
+    for i in range(m):
+        p, row = F.one(), []
+        for _ in range(m):
+            row.append(p)
+            p = F.mul(p, xs[i])
+        rows.append(row + [ys[i]])
+    for col in range(m):
+        piv = next(r for r in range(col, m) if rows[r][col] != F.zero())
+        rows[col], rows[piv] = rows[piv], rows[col]
+        s = F.inv(rows[col][col])
+        rows[col] = [F.mul(v, s) for v in rows[col]]
+        for r in range(m):
+            if r != col and rows[r][col] != F.zero():
+                f = rows[r][col]
+                rows[r] = [F.sub(a, F.mul(f, b)) for a, b in zip(rows[r], rows[col])]
+    return [rows[i][m] for i in range(m)]
+
+
+def to_coeffs(F, nodes, inf_index, values):
+    """EvalDomain(nodes, inf_index).to_coeffs(values) (domain.py:72-93); nodes
+    None stands for the naturals."""
+    if inf_index is None:
+        xs = nodes if nodes is not None else [F.from_int(i) for i in range(len(values))]
+        return interp(F, xs, values)
+    pos = inf_index % len(values)
+    v_inf = values[pos]
+    finite = values[:pos] + values[pos + 1:]
+    d = len(finite)
+    xs = nodes if nodes is not None else [F.from_int(i) for i in range(d)]
+    shifted = []
+    for x, y in zip(xs, finite):
+        xd = F.one()
+        for _ in range(d):
+            xd = F.mul(xd, x)
+        shifted.append(F.sub(y, F.mul(v_inf, xd)))
+    low = interp(F, xs, shifted) if d else []
+    return low + [v_inf]
+
+
+def _points(F, nodes, inf_index):
+    """Sample points of a domain in order; None marks infinity."""
+    pts = list(nodes)
+    if inf_index is not None:
+        pts.insert(inf_index % (len(nodes) + 1), None)
+    return pts
+
+
+def _sample(F, lo, hi, x):
+    d = F.sub(hi, lo)
+    return d if x is None else F.add(lo, F.mul(x, d))
+
+
+def _product_samples(F, state, pts):
+    h = len(state[0]) // 2
+    out = []
+    for x in pts:
+        acc = F.zero()
+        for j in range(h):
+            prod = F.one()
+            for tab in state:
+                prod = F.mul(prod, _sample(F, tab[j], tab[j + h], x))
+            acc = F.add(acc, prod)
+        out.append(acc)
+    return out
+
+
+def _wire_message(F, wire, samples_of, degree):
+    """The message of one round: samples_of(points) samples the round
+    polynomial (of the given degree) at a point list."""
+    kind = wire[0]
+    if kind == "values":
+        return samples_of([F.from_int(t) for t in range(degree + 1)])
+    if kind == "domain":
+        return samples_of(_points(F, wire[1], wire[2]))
+    if kind == "coeffs":
+        full = [None] + [F.from_int(t) for t in range(degree)]
+        return to_coeffs(F, None, 0, samples_of(full))
+    if kind == "compressed":
+        s = samples_of([F.zero(), None])
+        return [s[0], s[1]]
+    raise ValueError(kind)
+
+
+def wire_reduce(F, wire, claim, msg, r):
+    """The wire form's claim reduction (reduce.py:27-66)."""
+    kind = wire[0]
+    if kind == "values":
+        return lagrange_eval(F, msg, r)
+    if kind == "domain":
+        return eval_coeffs(F, to_coeffs(F, wire[1], wire[2], msg), r)
+    if kind == "coeffs":
+        return eval_coeffs(F, msg, r)
+    c0, c2 = msg
+    c1 = F.sub(F.sub(F.sub(claim, c0), c0), c2)
+    return eval_coeffs(F, [c0, c1, c2], r)
+
+
+def prove_product_wire(F, T, tables, claim, transcript: OracleTranscript, degree: int,
+                       wire) -> OracleProof:
+    """A product-summand stage whose rounds send `wire` messages:
+    StandardRound on a domain, CompressedProductRound, or coefficients."""
+    state = [[F.embed(x) for x in t] for t in tables]
+    n = log2_strict(len(state[0]))
+    msgs, words, point = [], [], []
+    for _ in range(n):
+        msg = _wire_message(F, wire, lambda pts: _product_samples(F, state, pts), degree)
+        transcript, rw, r = fs_hop(transcript, F, T, msg)
+        claim = wire_reduce(F, wire, claim, msg, r)
+        state = [_fold(F, tab, r) for tab in state]
+        msgs.append(msg)
+        words.append(rw)
+        point.append(r)
+    return OracleProof(msgs, words, point, claim, transcript, state)
+
+
+def prove_eq_wire(F, T, factors, w, claim, transcript: OracleTranscript, summand_degree: int,
+                  wire, eq_mass=None) -> OracleProof:
+    """EqPolyRound under fold_rounds with messages on `wire` (the reference's
+    default is ("domain", Û nodes, 0)); `eq_mass` starts the bound eq mass."""
+    state = [[F.embed(x) for x in t] for t in factors]
+    w = [F.embed(x) for x in w]
+    l = len(w)
+    l_half = l // 2
+    eq_l_list = suffix_eq_tables(F, w[:l_half])
+    eq_r_list = suffix_eq_tables(F, w[l_half:])
+    eq_prev = F.one() if eq_mass is None else eq_mass
+    msgs, words, point = [], [], []
+    for _ in range(l):
+        nlen = len(state[0])
+        i = l - log2_strict(nlen) + 1
+        rest = l - i
+        if rest == 0:
+            eq_wl, eq_wr = None, [F.one()]
+        elif i < l_half:
+            eq_wl, eq_wr = eq_l_list[(l_half - i) - 1], eq_r_list[-1]
+        else:
+            eq_wl, eq_wr = None, eq_r_list[rest - 1]
+        h = nlen // 2
+        w_i = w[i - 1]
+        l1 = F.mul(eq_prev, w_i)
+        l0 = F.sub(eq_prev, l1)
+        ldiff = F.sub(l1, l0)
+
+        def samples_of(pts, state=state, eq_wl=eq_wl, eq_wr=eq_wr, h=h, l0=l0, ldiff=ldiff):
+            out = []
+            for x in pts:
+                acc = F.zero()
+                for y in range(h):
+                    prod = F.one()
+                    for tab in state:
+                        prod = F.mul(prod, _sample(F, tab[y], tab[y + h], x))
+                    if eq_wl is not None:
+                        nxr = len(eq_wr)
+                        wt = F.mul(eq_wl[y // nxr], eq_wr[y % nxr])
+                    else:
+                        wt = eq_wr[y]
+                    acc = F.add(acc, F.mul(prod, wt))
+                lx = ldiff if x is None else F.add(l0, F.mul(x, ldiff))
+                out.append(F.mul(lx, acc))
+            return out
+
+        msg = _wire_message(F, wire, samples_of, summand_degree + 1)
+        transcript, rw, r = fs_hop(transcript, F, T, msg)
+        claim = wire_reduce(F, wire, claim, msg, r)
+        state = [_fold(F, tab, r) for tab in state]
+        eq_prev = F.mul(eq_prev, eq_factor(F, r, w_i))
+        msgs.append(msg)
+        words.append(rw)
+        point.append(r)
+    return OracleProof(msgs, words, point, claim, transcript, state, eq_prev)

@@ -128,3 +128,43 @@ def run_py(inst) -> tuple[OracleResult, OS.OracleProof]:
     state = pf.transcript.state()
     h, pend, tot = state if inst.transcript_kind == "sha256" else ((), b"", 0)
     return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot, state), pf
+
+
+def oracle_wire(inst, wire_py) -> tuple:
+    """(F, wire) of the oracle for a package wire description: ("values",),
+    ("domain", EvalDomain), ("coeffs",) or ("compressed",)."""
+    cf = OFIELD[inst.chal.name]
+    if wire_py[0] != "domain":
+        return cf, wire_py
+    dom = wire_py[1]
+    nodes = [cf.embed(OFIELD[dom.nodes.field.name].from_words(
+        [int(w) for w in dom.nodes.words[i]])) for i in range(dom.nodes.shape[0])]
+    return cf, ("domain", nodes, dom.inf_index)
+
+
+def run_py_wire(inst, wire_py, eq_mass=None) -> tuple[OracleResult, OS.OracleProof]:
+    """The Python oracle's stage with `wire_py` messages (small instances)."""
+    tf, cf, trf = (OFIELD[inst.tables.field.name], OFIELD[inst.chal.name],
+                   OFIELD[inst.transcript_field.name])
+    tabs = [[tf.from_words([int(w) for w in inst.tables.words[q, x]])
+             for x in range(inst.tables.shape[1])] for q in range(inst.tables.shape[0])]
+    claim = cf.from_words([int(w) for w in inst.claim.words.reshape(-1)])
+    tcls = OT.OracleBlake3Transcript if inst.transcript_kind == "blake3" else OT.OracleTranscript
+    tr = tcls.new(inst.label, trf.item_bytes)
+    _, wire = oracle_wire(inst, wire_py)
+    mass = None if eq_mass is None else cf.from_words([int(w) for w in eq_mass.words.reshape(-1)])
+    if inst.mode == "zerocheck":
+        z = [cf.from_words([int(w) for w in inst.z.words[i]]) for i in range(inst.n)]
+        pf = OS.prove_eq_wire(cf, trf, tabs, z, claim, tr, inst.degree, wire, mass)
+    else:
+        pf = OS.prove_product_wire(cf, trf, tabs, claim, tr, inst.degree, wire)
+    wd = np.uint64 if cf is OF.GOLDILOCKS else np.uint32
+    msgs = np.array([[cf.to_words(m) for m in row] for row in pf.msgs], dtype=wd)
+    point = np.array(pf.point_words, dtype=wd)
+    value = np.array(cf.to_words(pf.value), dtype=wd)
+    fin = [cf.to_words(t[0]) for t in pf.final_tables]
+    if inst.mode == "zerocheck":
+        fin.append(cf.to_words(pf.eq_prev))
+    state = pf.transcript.state()
+    h, pend, tot = state if inst.transcript_kind == "sha256" else ((), b"", 0)
+    return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot, state), pf

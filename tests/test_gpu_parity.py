@@ -22,6 +22,7 @@ from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, array, babybear_mont, babybearx4_mont, goldilocks
 from zorch_amd.prove import fold_rounds
 from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+from zorch_amd.sumcheck.domain import natural_domain
 from zorch_amd.sumcheck.eq.eq_poly import EqPolyRound
 from zorch_amd.sumcheck.eq.stage import EqPolyProver, EqPolyVerifier, EqPolyWitness, EqSumClaim
 from zorch_amd.sumcheck.prover import ProductSummand, StandardRound, initial_claim
@@ -84,7 +85,9 @@ def test_round_by_round_matches_golden_fixture(name):
     t0 = W.new_transcript(inst)
     pol = ChallengePolicy(inst.chal)
     if inst.mode == "zerocheck":
-        rnd = EqPolyRound(ProductSummand(inst.degree), inst.z, challenges=pol)
+        # The natural domain, as EqPolyProver passes it (eq/stage.py:70-76).
+        rnd = EqPolyRound(ProductSummand(inst.degree), inst.z,
+                          natural_domain(inst.degree + 1, inst.chal), challenges=pol)
     else:
         rnd = StandardRound(ProductSummand(inst.degree), challenges=pol)
     carry, t, msgs = fold_rounds(rnd, initial_claim(inst.tables, inst.claim, inst.n), t0, inst.n)
@@ -228,7 +231,8 @@ def test_chained_rounds_round_by_round():
     inst = W.zerocheck_bb4(17, (102, 103), 104, 15)
     whole = _prove(inst)
     t0 = W.new_transcript(inst)
-    rnd = EqPolyRound(ProductSummand(inst.degree), inst.z, challenges=ChallengePolicy(inst.chal))
+    rnd = EqPolyRound(ProductSummand(inst.degree), inst.z, natural_domain(inst.degree + 1, inst.chal),
+                      challenges=ChallengePolicy(inst.chal))
     carry, t, msgs = fold_rounds(rnd, initial_claim(inst.tables, inst.claim, inst.n), t0, inst.n)
     assert [m.words.tolist() for m in msgs] == whole.reduction_proof.words.tolist()
     assert carry.claim.value.words.tolist() == whole.reduced_claim.value.words.tolist()

@@ -17,6 +17,9 @@ ZS_OK, ZS_EINVAL, ZS_ENOMEM, ZS_EHIP, ZS_EBUDGET, ZS_ENODEV = 0, 1, 2, 3, 4, 5
 ZS_F_BABYBEAR, ZS_F_BABYBEAR4, ZS_F_GOLDILOCKS = 1, 2, 3
 ZS_TR_SHA256, ZS_TR_BLAKE3 = 1, 2
 ZS_MODE_PRODUCT, ZS_MODE_EQ = 1, 2
+ZS_MAX_ROUNDS = 31
+ZS_WIRE_VALUES, ZS_WIRE_DOMAIN, ZS_WIRE_COEFFS, ZS_WIRE_COMPRESSED = 0, 1, 2, 3
+ZS_OP_ADD, ZS_OP_SUB, ZS_OP_MUL = 1, 2, 3
 
 
 class Sha256StateC(ctypes.Structure):
@@ -61,6 +64,16 @@ class TranscriptC(ctypes.Structure):
         return out
 
 
+class WireC(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("num_nodes", ctypes.c_uint32),
+        ("has_inf", ctypes.c_uint32),
+        ("inf_pos", ctypes.c_uint32),
+        ("nodes", ctypes.c_void_p),
+    ]
+
+
 class ProveArgsC(ctypes.Structure):
     _fields_ = [
         ("table_field", ctypes.c_uint32),
@@ -74,6 +87,8 @@ class ProveArgsC(ctypes.Structure):
         ("reserved", ctypes.c_uint32),
         ("eq_point", ctypes.c_void_p),
         ("claim", ctypes.c_void_p),
+        ("wire", ctypes.POINTER(WireC)),
+        ("eq_mass", ctypes.c_void_p),
     ]
 
 
@@ -120,6 +135,7 @@ class ServerStatsC(ctypes.Structure):
 assert ctypes.sizeof(Sha256StateC) == 112
 assert ctypes.sizeof(Blake3StateC) == 632
 assert ctypes.sizeof(TranscriptC) == 648
+assert ctypes.sizeof(ProveArgsC) == 72
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -151,6 +167,9 @@ SIGNATURES = {
     "zs_transcript_absorb": (_INT, [ctypes.POINTER(TranscriptC), _P, _SZ]),
     "zs_transcript_squeeze": (_INT, [ctypes.POINTER(TranscriptC), _SZ, _P]),
     "zs_transcript_digest": (_INT, [ctypes.POINTER(TranscriptC), _P]),
+    "zs_prove_check": (_INT, [ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
+                              ctypes.POINTER(_U32)]),
+    "zs_field_op": (_INT, [_P, _U32, _U32, _P, _P, _P, _SZ]),
     "zs_prove": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                         ctypes.POINTER(ProofC)]),
     "zs_prove_batch": (_INT, [_P, _SZ, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),

@@ -133,7 +133,7 @@ class BatchGroupImpl final : public BatchGroup {
       uint8_t* rec = h + m.rec;
       Ctl<FC>& c = *(Ctl<FC>*)rec;
       c = proto;
-      set_ctl_proof<FC>(c, m.t, m.a.claim);
+      if ((rc = set_ctl_proof<FC>(c, m.t, m.a, NPTS))) return rc;
       if (EQ) fill_eq_point<FC>((TC*)(rec + o_eqp(m.n)), (TC*)(rec + o_eqab(m.n)), m.a.eq_point, m.n);
       const size_t N = (size_t)1 << m.n;
       const uint32_t l_half = m.n / 2;
@@ -223,7 +223,7 @@ class BatchGroupImpl final : public BatchGroup {
       const uint8_t* rec = h + m.rec;
       const Ctl<FC>& c = *(const Ctl<FC>*)rec;
       zs_proof& o = outs[m.index];
-      if (o.msgs) memcpy(o.msgs, rec + o_msgs(), (size_t)m.n * NPTS * sizeof(TC));
+      if (o.msgs) copy_msg_rows(o.msgs, rec + o_msgs(), m.n, NPTS, c.mlen, sizeof(TC));
       if (o.point) memcpy(o.point, rec + o_point(m.n), (size_t)m.n * sizeof(TC));
       if (o.value) memcpy(o.value, &c.claim, sizeof(TC));
       if (o.final_tables)
@@ -301,14 +301,11 @@ class BatchGroupImpl final : public BatchGroup {
   void* host_ = nullptr;
 };
 
-template <class FT, class FC>
-BatchGroup* make_typed_batch(uint32_t k, uint32_t npts, bool eq) {
-#define ZS_CASE(KK, EQQ)                                                    \
-  if (k == KK && eq == EQQ && npts == (uint32_t)(KK + 1 + (EQQ ? 1 : 0))) \
-    return new BatchGroupImpl<FT, FC, KK, KK + 1 + (EQQ ? 1 : 0), EQQ>();
-  ZS_CASE(1, false) ZS_CASE(2, false) ZS_CASE(3, false) ZS_CASE(4, false)
-  ZS_CASE(1, true) ZS_CASE(2, true) ZS_CASE(3, true) ZS_CASE(4, true)
-#undef ZS_CASE
+// The BatchGroupImpl of K factor tables for a message of npts values (K + 1, or K + 2 with the eq factor).
+template <class FT, class FC, int K>
+BatchGroup* make_typed_batch_k(uint32_t npts, bool eq) {
+  if (eq && npts == (uint32_t)(K + 2)) return new BatchGroupImpl<FT, FC, K, K + 2, true>();
+  if (!eq && npts == (uint32_t)(K + 1)) return new BatchGroupImpl<FT, FC, K, K + 1, false>();
   return nullptr;
 }
 

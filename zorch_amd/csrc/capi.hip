@@ -31,11 +31,16 @@ int hip_fail(hipError_t e, const char* what) {
 
 // ------------------------------------------------------------------- pool
 DevicePool::~DevicePool() {
-  trim();
+  trim_locked();
   for (auto& kv : sizes_) (void)hipFree(kv.first);
 }
 
 void DevicePool::trim() {
+  std::lock_guard<std::mutex> g(mu_);
+  trim_locked();
+}
+
+void DevicePool::trim_locked() {
   for (auto& kv : free_) {
     (void)hipFree(kv.second);
     sizes_.erase(kv.second);
@@ -46,6 +51,7 @@ void DevicePool::trim() {
 
 int DevicePool::alloc(size_t bytes, void** out) {
   bytes = (bytes + 4095) & ~(size_t)4095;
+  std::lock_guard<std::mutex> g(mu_);
   auto it = free_.find(bytes);
   if (it != free_.end()) {
     *out = it->second;
@@ -58,11 +64,11 @@ int DevicePool::alloc(size_t bytes, void** out) {
     return fail(ZS_EBUDGET, "device memory budget exceeded: " + std::to_string(live_ + bytes) +
                                 " > " + std::to_string(budget_) + " bytes");
   }
-  if (live_ + cached_ + bytes > budget_) trim();
+  if (live_ + cached_ + bytes > budget_) trim_locked();
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
-    trim();
+    trim_locked();
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
   }
@@ -73,6 +79,7 @@ int DevicePool::alloc(size_t bytes, void** out) {
 }
 
 void DevicePool::release(void* p) {
+  std::lock_guard<std::mutex> g(mu_);
   auto it = sizes_.find(p);
   if (it == sizes_.end()) return;
   live_ -= it->second;
@@ -86,6 +93,7 @@ HostPool::~HostPool() {
 
 int HostPool::alloc(size_t bytes, void** out) {
   bytes = (bytes + 4095) & ~(size_t)4095;
+  std::lock_guard<std::mutex> g(mu_);
   auto it = free_.find(bytes);
   if (it != free_.end()) {
     *out = it->second;
@@ -101,6 +109,7 @@ int HostPool::alloc(size_t bytes, void** out) {
 }
 
 void HostPool::release(void* p) {
+  std::lock_guard<std::mutex> g(mu_);
   auto it = sizes_.find(p);
   if (it == sizes_.end()) return;
   free_.emplace(it->second, p);
@@ -131,7 +140,8 @@ static bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, siz
 // Base prime field of a field id.
 static uint32_t base_of(uint32_t f) { return f == ZS_F_BABYBEAR4 ? ZS_F_BABYBEAR : f; }
 
-int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts) {
+int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts,
+                  uint32_t* mlen) {
   if (!field_bytes(a.table_field) || !field_bytes(a.chal_field))
     return fail(ZS_EINVAL, "unknown field id");
   const bool ok_pair = (a.table_field == a.chal_field) ||
@@ -143,7 +153,9 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
     return fail(ZS_EINVAL, "between 1 and 4 stacked factor tables are supported");
   if (a.degree != a.num_tables)
     return fail(ZS_EINVAL, "the product summand's degree must equal the number of stacked tables");
-  if (a.rounds < 1 || a.rounds > 33) return fail(ZS_EINVAL, "rounds must be in [1, 33]");
+  // Table lengths of a round (2^(n - r) entries) are 32-bit in the kernels.
+  if (a.rounds < 1 || a.rounds > ZS_MAX_ROUNDS)
+    return fail(ZS_EINVAL, "rounds must be in [1, " + std::to_string(ZS_MAX_ROUNDS) + "]");
   if (!a.tables || !a.claim) return fail(ZS_EINVAL, "null tables or claim");
   if (a.mode == ZS_MODE_EQ && !a.eq_point) return fail(ZS_EINVAL, "eq mode needs an equality point");
   if (t) {
@@ -154,32 +166,51 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
     if (t->item_bytes != field_bytes(tf)) return fail(ZS_EINVAL, "transcript item size mismatch");
   }
   *npts = a.degree + 1 + (a.mode == ZS_MODE_EQ ? 1 : 0);
+  const uint32_t m = wire_width(a.wire, *npts, a.mode == ZS_MODE_PRODUCT && a.num_tables == 2);
+  if (!m) return ZS_EINVAL;
+  if ((size_t)m * field_bytes(a.chal_field) > 96)
+    return fail(ZS_EINVAL, "a round message is at most 96 bytes");
+  if (mlen) *mlen = m;
   return ZS_OK;
 }
 
-BatchGroup* make_batch_group(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_batch_group_bb(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_batch_group_bb_bb4(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_batch_group_bb4(k, npts, eq);
-  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_batch_group_gl(k, npts, eq);
+// Dispatch to the unit of a field pair and factor count (nullptr: unsupported).
+#define ZS_DISPATCH(WHAT)                                                              \
+  if (k < 1 || k > 4) return nullptr;                                                  \
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) {                                    \
+    static decltype(&WHAT##_bb_k1) const f[4] = {WHAT##_bb_k1, WHAT##_bb_k2,            \
+                                                 WHAT##_bb_k3, WHAT##_bb_k4};           \
+    return f[k - 1](npts, eq);                                                         \
+  }                                                                                    \
+  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) {                                   \
+    static decltype(&WHAT##_bb_bb4_k1) const f[4] = {WHAT##_bb_bb4_k1, WHAT##_bb_bb4_k2, \
+                                                     WHAT##_bb_bb4_k3, WHAT##_bb_bb4_k4}; \
+    return f[k - 1](npts, eq);                                                         \
+  }                                                                                    \
+  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) {                                  \
+    static decltype(&WHAT##_bb4_k1) const f[4] = {WHAT##_bb4_k1, WHAT##_bb4_k2,         \
+                                                  WHAT##_bb4_k3, WHAT##_bb4_k4};        \
+    return f[k - 1](npts, eq);                                                         \
+  }                                                                                    \
+  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) {                                \
+    static decltype(&WHAT##_gl_k1) const f[4] = {WHAT##_gl_k1, WHAT##_gl_k2,            \
+                                                 WHAT##_gl_k3, WHAT##_gl_k4};           \
+    return f[k - 1](npts, eq);                                                         \
+  }                                                                                    \
   return nullptr;
+
+BatchGroup* make_batch_group(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
+  ZS_DISPATCH(make_batch_group)
 }
 
 ServeKind* make_serve_kind(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_serve_kind_bb(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_serve_kind_bb_bb4(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_serve_kind_bb4(k, npts, eq);
-  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_serve_kind_gl(k, npts, eq);
-  return nullptr;
+  ZS_DISPATCH(make_serve_kind)
 }
 
 Instance* make_instance(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, bool eq) {
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR) return make_instance_bb(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR && cf == ZS_F_BABYBEAR4) return make_instance_bb_bb4(k, npts, eq);
-  if (tf == ZS_F_BABYBEAR4 && cf == ZS_F_BABYBEAR4) return make_instance_bb4(k, npts, eq);
-  if (tf == ZS_F_GOLDILOCKS && cf == ZS_F_GOLDILOCKS) return make_instance_gl(k, npts, eq);
-  return nullptr;
+  ZS_DISPATCH(make_instance)
 }
+#undef ZS_DISPATCH
 
 static int set_device(zs_ctx* ctx) {
   hipError_t e = hipSetDevice(ctx->device);
@@ -222,6 +253,40 @@ static int verify_round_typed(zs_transcript* t, const void* claim_p, const void*
   if (reduced_out) memcpy(reduced_out, &red, sizeof(T));
   *ok = F::eq(claim, F::add(msg[0], msg[1])) ? 1 : 0;
   return ZS_OK;
+}
+
+// One field operation per thread (zs_field_op).
+template <class F>
+__global__ void field_op_kernel(uint32_t op, const typename F::T* a, const typename F::T* b,
+                                typename F::T* out, size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const typename F::T x = F::canon(a[i]), y = F::canon(b[i]);
+  out[i] = op == ZS_OP_ADD ? F::add(x, y) : op == ZS_OP_SUB ? F::sub(x, y) : F::mul(x, y);
+}
+
+template <class F>
+static int field_op_typed(zs_ctx* ctx, uint32_t op, const void* a, const void* b, void* out,
+                          size_t count) {
+  using T = typename F::T;
+  const size_t bytes = count * sizeof(T);
+  void* d = nullptr;
+  int rc = ctx->pool->alloc(3 * bytes, &d);
+  if (rc) return rc;
+  T* da = (T*)d;
+  T* db = da + count;
+  T* dout = db + count;
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    field_op_kernel<F><<<(uint32_t)((count + 255) / 256), 256, 0, s>>>(op, da, db, dout, count);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  ctx->pool->release(d);
+  return e == hipSuccess ? ZS_OK : hip_fail(e, "zs_field_op");
 }
 
 }  // namespace zs
@@ -502,6 +567,13 @@ int zs_transcript_digest(const zs_transcript* t, uint8_t out[32]) {
 }
 
 // ------------------------------------------------------------------ prove
+int zs_prove_check(const zs_prove_args* args, const zs_transcript* transcript,
+                   uint32_t* msg_len) {
+  if (!args || !msg_len) return fail(ZS_EINVAL, "null argument");
+  uint32_t npts = 0;
+  return validate_args(*args, transcript, &npts, msg_len);
+}
+
 int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript, zs_proof* out) {
   if (!ctx || !args || !transcript || !out) return fail(ZS_EINVAL, "null argument");
   uint32_t npts = 0;
@@ -614,6 +686,22 @@ void zs_state_destroy(zs_state* st) {
   (void)hipSetDevice(st->ctx->device);
   (void)hipStreamSynchronize(st->ctx->stream);
   delete st;
+}
+
+int zs_field_op(zs_ctx* ctx, uint32_t field, uint32_t op, const void* a, const void* b,
+                void* out, size_t count) {
+  if (!ctx || (count && (!a || !b || !out))) return fail(ZS_EINVAL, "null argument");
+  if (op != ZS_OP_ADD && op != ZS_OP_SUB && op != ZS_OP_MUL) return fail(ZS_EINVAL, "unknown op");
+  if (count == 0) return ZS_OK;
+  if (count > ((size_t)1 << 28)) return fail(ZS_EINVAL, "at most 2^28 operations per call");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  switch (field) {
+    case ZS_F_BABYBEAR: return field_op_typed<BB>(ctx, op, a, b, out, count);
+    case ZS_F_BABYBEAR4: return field_op_typed<BB4>(ctx, op, a, b, out, count);
+    case ZS_F_GOLDILOCKS: return field_op_typed<GL>(ctx, op, a, b, out, count);
+  }
+  return fail(ZS_EINVAL, "unknown field id");
 }
 
 int zs_verify_round(uint32_t chal_field, zs_transcript* t, const void* claim, const void* msg,

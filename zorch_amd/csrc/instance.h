@@ -1,6 +1,7 @@
 // Typed proof instance: device buffers, uploads, the per-round launch
 // sequence and the downloads. Instantiated per field combination in
-// inst_*.hip (one translation unit each, compiled in parallel).
+// inst_<fields>_k<K>.hip (one translation unit per field pair and factor
+// count, compiled in parallel).
 //
 // Device block layout (one pool allocation, 256-byte aligned regions):
 //   io region:  control block | messages | raw point | final values | slot rows |
@@ -17,6 +18,7 @@
 
 #include "kernels.h"
 #include "runtime.h"
+#include "wire.h"
 
 namespace zs {
 
@@ -49,9 +51,10 @@ void init_ctl_consts(Ctl<FC>& c) {
   }
 }
 
-// The per-proof part of a control block: transcript, claim, running state.
+// The per-proof part of a control block: transcript, claim, running state,
+// and the maps of the request's message wire form (ZS_OK or ZS_EINVAL).
 template <class FC>
-void set_ctl_proof(Ctl<FC>& c, const zs_transcript& t, const void* claim) {
+int set_ctl_proof(Ctl<FC>& c, const zs_transcript& t, const zs_prove_args& a, uint32_t npts) {
   static_assert(sizeof(TrState) == sizeof(zs_transcript) - offsetof(zs_transcript, sha),
                 "transcript state layout");
   memcpy(&c.tr, &t.sha, sizeof(TrState));
@@ -59,9 +62,19 @@ void set_ctl_proof(Ctl<FC>& c, const zs_transcript& t, const void* claim) {
   c.item_bytes = t.item_bytes;
   c.limbs = (uint32_t)(sizeof(typename FC::T) / t.item_bytes);
   c.counter = 0;
-  c.claim = load_host_el<FC>(claim, 0);
+  c.claim = load_host_el<FC>(a.claim, 0);
   c.r = FC::zero();
-  c.eq_prev = FC::one();
+  c.eq_prev = a.eq_mass ? load_host_el<FC>(a.eq_mass, 0) : FC::one();
+  return set_ctl_wire<FC>(c, npts, a.wire);
+}
+
+// Copy `rounds` message rows of the device layout (npts elements a row) into
+// the caller's layout of mlen elements a row.
+inline void copy_msg_rows(void* dst, const void* src, uint32_t rounds, uint32_t npts,
+                          uint32_t mlen, size_t el_bytes) {
+  for (uint32_t i = 0; i < rounds; ++i)
+    memcpy((uint8_t*)dst + (size_t)i * mlen * el_bytes,
+           (const uint8_t*)src + (size_t)i * npts * el_bytes, (size_t)mlen * el_bytes);
 }
 
 // Eq point and the coefficients of l_i(t) = eq_prev ((1 - w_i) + t (2 w_i - 1)).
@@ -150,7 +163,8 @@ class InstanceImpl final : public Instance {
     memset(h, 0, io_bytes_);
     Ctl<FC>& c = *(Ctl<FC>*)(h + o_ctl_);
     init_ctl_consts<FC, K, NPTS>(c);
-    set_ctl_proof<FC>(c, t, a.claim);
+    if ((rc = set_ctl_proof<FC>(c, t, a, NPTS))) return rc;
+    mlen = c.mlen;
     if (EQ) fill_eq_point<FC>((TC*)(h + o_eqp), (TC*)(h + o_eqab), a.eq_point, n_);
     ZS_HIP(hipMemcpyAsync(b, h, io_bytes_, hipMemcpyHostToDevice, s));
     if (EQ) {
@@ -383,7 +397,7 @@ class InstanceImpl final : public Instance {
     const uint8_t* h = (const uint8_t*)host_;
     const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
     if (c.fault) return fail(ZS_EHIP, "chained rounds: a device wait reached its limit");
-    if (out->msgs) memcpy(out->msgs, h + o_msgs_, (size_t)n_ * NPTS * sizeof(TC));
+    if (out->msgs) copy_msg_rows(out->msgs, h + o_msgs_, n_, NPTS, mlen, sizeof(TC));
     if (out->point) memcpy(out->point, h + o_point_, (size_t)n_ * sizeof(TC));
     if (out->value) memcpy(out->value, &c.claim, sizeof(TC));
     if (out->final_tables) memcpy(out->final_tables, h + o_final_, (K + (EQ ? 1 : 0)) * sizeof(TC));
@@ -410,7 +424,7 @@ class InstanceImpl final : public Instance {
     const uint8_t* h = (const uint8_t*)host_;
     const Ctl<FC>& c = *(const Ctl<FC>*)(h + o_ctl_);
     if (c.fault) return fail(ZS_EHIP, "chained rounds: a device wait reached its limit");
-    if (msg) memcpy(msg, h + o_msgs_ + (size_t)round * NPTS * sizeof(TC), NPTS * sizeof(TC));
+    if (msg) memcpy(msg, h + o_msgs_ + (size_t)round * NPTS * sizeof(TC), mlen * sizeof(TC));
     if (chal) memcpy(chal, h + o_point_ + (size_t)round * sizeof(TC), sizeof(TC));
     if (claim) memcpy(claim, &c.claim, sizeof(TC));
     if (t) memcpy(&t->sha, &c.tr, sizeof(TrState));
@@ -458,14 +472,11 @@ class InstanceImpl final : public Instance {
   StageArgs<FT, FC> args_{};
 };
 
-template <class FT, class FC>
-Instance* make_typed(uint32_t k, uint32_t npts, bool eq) {
-#define ZS_CASE(KK, EQQ)                                                    \
-  if (k == KK && eq == EQQ && npts == (uint32_t)(KK + 1 + (EQQ ? 1 : 0))) \
-    return new InstanceImpl<FT, FC, KK, KK + 1 + (EQQ ? 1 : 0), EQQ>();
-  ZS_CASE(1, false) ZS_CASE(2, false) ZS_CASE(3, false) ZS_CASE(4, false)
-  ZS_CASE(1, true) ZS_CASE(2, true) ZS_CASE(3, true) ZS_CASE(4, true)
-#undef ZS_CASE
+// The InstanceImpl of K factor tables for a message of npts values (K + 1, or K + 2 with the eq factor).
+template <class FT, class FC, int K>
+Instance* make_typed_k(uint32_t npts, bool eq) {
+  if (eq && npts == (uint32_t)(K + 2)) return new InstanceImpl<FT, FC, K, K + 2, true>();
+  if (!eq && npts == (uint32_t)(K + 1)) return new InstanceImpl<FT, FC, K, K + 1, false>();
   return nullptr;
 }
 

@@ -82,7 +82,20 @@ struct Ctl {
   typename FC::T lag_inv[MAXPTS];  // 1 / prod_{j != t} (t - j) over {0..npts-1}
   typename FC::T nodes[MAXPTS];    // t = 0..npts-1 as field elements
   typename FC::T ext_coef[MAXPTS]; // (-1)^(K-i) C(K+1, i): t(K+1) from t(0..K)
+  // Round-message wire form (the C ABI's ZS_WIRE_*), with its maps built on
+  // the host: message element j is sum_i wmat[j][i] s(i) over the natural
+  // values s(0..npts-1), and the reduced claim is sum_k r^k c_k with
+  // c_k = sum_j cmat[k][j] msg_j + cclaim[k] claim. WIRE_VALUES (the natural
+  // values themselves) skips both maps and reduces by Lagrange.
+  uint32_t wire;
+  uint32_t mlen;         // message elements (<= npts)
+  uint32_t ncoef;        // coefficients of the reduced polynomial
+  uint32_t claim_every;  // the reduction reads the running claim: reduce every round
+  typename FC::T wmat[MAXPTS][MAXPTS];
+  typename FC::T cmat[MAXPTS][MAXPTS];
+  typename FC::T cclaim[MAXPTS];
 };
+constexpr uint32_t WIRE_VALUES = 0, WIRE_DOMAIN = 1, WIRE_COEFFS = 2, WIRE_COMPRESSED = 3;
 
 template <class FT, class FC>
 struct StageArgs {
@@ -676,6 +689,10 @@ struct FinishLds : HopLds {
   // Lagrange denominators; the current round's eq line coefficients.
   typename FC::T nodes[MAXPTS], ext_coef[MAXPTS], lag_inv[MAXPTS];
   typename FC::T eqA, eqB;
+  // The wire form's maps (Ctl::wire ...); snat holds the natural values.
+  uint32_t wire, mlen, ncoef, claim_every;
+  typename FC::T wmat[MAXPTS][MAXPTS], cmat[MAXPTS][MAXPTS], cclaim[MAXPTS];
+  alignas(16) typename FC::T snat[MAXPTS];
 };
 
 // Control block -> finisher LDS (wave 0, all lanes).
@@ -703,6 +720,19 @@ __device__ __forceinline__ void fin_load(FinishLds<FC>& sm, const Ctl<FC>* ctl) 
     sm.nodes[lane] = ctl->nodes[lane];
     sm.ext_coef[lane] = ctl->ext_coef[lane];
     sm.lag_inv[lane] = ctl->lag_inv[lane];
+    sm.cclaim[lane] = ctl->cclaim[lane];
+  }
+  if (lane == 0) {
+    sm.wire = ctl->wire;
+    sm.mlen = ctl->mlen;
+    sm.ncoef = ctl->ncoef;
+    sm.claim_every = ctl->claim_every;
+  }
+  if (ctl->wire != WIRE_VALUES) {
+    for (uint32_t i = lane; i < (uint32_t)(MAXPTS * MAXPTS); i += 64u) {
+      (&sm.wmat[0][0])[i] = (&ctl->wmat[0][0])[i];
+      (&sm.cmat[0][0])[i] = (&ctl->cmat[0][0])[i];
+    }
   }
   wave_sync();
 }
@@ -782,20 +812,36 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
     } else {
       msg = tv;
     }
-    if (lane < (uint32_t)NPTS) {
-      a.msgs[(size_t)round * NPTS + lane] = msg;
-      sm.msg[lane] = msg;
+    if (sm.wire == WIRE_VALUES) {
+      if (lane < (uint32_t)NPTS) {
+        a.msgs[(size_t)round * NPTS + lane] = msg;
+        sm.msg[lane] = msg;
+      }
+    } else {
+      // Another wire form: element j of the message is row j of the wire
+      // map applied to the natural values.
+      if (lane < (uint32_t)NPTS) sm.snat[lane] = msg;
+      wave_sync();
+      const uint32_t j = t < sm.mlen ? t : 0u;
+      T m = FC::zero();
+#pragma unroll
+      for (int i = 0; i < NPTS; ++i) m = FC::add(m, FC::mul(sm.wmat[j][i], sm.snat[i]));
+      if (lane < sm.mlen) {
+        a.msgs[(size_t)round * NPTS + lane] = m;
+        sm.msg[lane] = m;
+      }
     }
   }
   wave_sync();
   if (lane == 0) trace_mark(a.trace, round, 4);
   static_assert(NPTS * EB + 32 <= (int)sizeof(HopLds::buf), "hop framing buffer");
+  const uint32_t mlen = sm.wire == WIRE_VALUES ? (uint32_t)NPTS : sm.mlen;
   if constexpr (HOP == HOP_INLINE) {
-    fs_hop_body(sm, reinterpret_cast<const uint8_t*>(sm.msg), (uint32_t)(NPTS * EB),
+    fs_hop_body(sm, reinterpret_cast<const uint8_t*>(sm.msg), mlen * (uint32_t)EB,
                 ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   } else {
     fs_hop<HOP>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
-                (uint32_t)(NPTS * EB), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+                mlen * (uint32_t)EB, ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   }
   if (lane == 0) trace_mark(a.trace, round, 5);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
@@ -803,7 +849,9 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   T eq_next = FC::zero();
   if constexpr (EQ) eq_next = FC::mul(sm.eq_prev, FC::add(sm.eqB, FC::mul(r, sm.eqA)));
   T claim = FC::zero();
-  if (need_claim) {
+  const bool wire_values = sm.wire == WIRE_VALUES;
+  need_claim = need_claim || sm.claim_every != 0;
+  if (need_claim && wire_values) {
     // s(r) = sum_t s(t) L_t(r) with L_t(r) = lag_inv[t] prod_{j != t} (r - j):
     // lane t forms term t; lanes past the nodes add zero.
     T basis = sm.lag_inv[tn];
@@ -814,6 +862,14 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
     }
     const T msg = t < (uint32_t)NPTS ? sm.msg[tn] : FC::zero();
     claim = lanes8_sum<FC>(FC::mul(msg, basis));
+  } else if (need_claim) {
+    // The wire form's reduction: lane k forms c_k r^k.
+    const uint32_t kk = t < sm.ncoef ? t : 0u;
+    T c = FC::mul(sm.cclaim[kk], sm.claim);
+    for (uint32_t j = 0; j < sm.mlen; ++j) c = FC::add(c, FC::mul(sm.cmat[kk][j], sm.msg[j]));
+    T rk = FC::one();
+    for (uint32_t i = 0; i < kk; ++i) rk = FC::mul(rk, r);
+    claim = lanes8_sum<FC>(t < sm.ncoef ? FC::mul(c, rk) : FC::zero());
   }
   wave_sync();
   if (lane == 0) {
@@ -2645,10 +2701,9 @@ static __device__ __attribute__((noinline)) void batch_tail(
                                            *(TailLds<FC, K + 1, PASS_THREADS>*)sh);
 }
 
-// RT: the round type of every item of the launch: 0 = round 0 (input tables,
-// no fold), 1 = round 1 (fold of the input tables), 2 = later rounds (fold in
-// the challenge field); 3 = mixed, chosen per item (proofs that joined a
-// continuous batch at different steps).
+// RT: the round type of every pass item of the launch: 0 = round 0 (input
+// tables, no fold), 1 = round 1 (fold of the input tables), 2 = later rounds
+// (fold in the challenge field). Tail items may ride with any of them.
 template <class FT, class FC, int K, bool EQ, int RT>
 __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
     const StageArgs<FT, FC>* __restrict__ args, const BatchItem* __restrict__ items,
@@ -2667,14 +2722,14 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
   const uint32_t blk = blockIdx.x - it.block_off;
   const uint32_t L = 1u << (a.n - round);
   const bool need_claim = round + 1 == a.n;
-  const int rt = RT == 3 ? (round == 0 ? 0 : round == 1 ? 1 : 2) : RT;
-  if ((RT == 0 || RT == 3) && rt == 0) {
+  static_assert(RT >= 0 && RT <= 2, "round type");
+  if constexpr (RT == 0) {
     round_block<FT, FT, FC, K, EQ, false>(a, a.in, a.in_stride, L, round, need_claim, blk,
                                           it.blocks, sh.round);
-  } else if ((RT == 1 || RT == 3) && rt == 1) {
+  } else if constexpr (RT == 1) {
     round_block<FT, FT, FC, K, EQ, true>(a, a.in, a.in_stride, L, round, need_claim, blk,
                                          it.blocks, sh.round);
-  } else if (RT >= 2) {
+  } else {
     round_block<FT, FC, FC, K, EQ, true>(a, a.work, a.work_stride, L, round, need_claim, blk,
                                          it.blocks, sh.round);
   }

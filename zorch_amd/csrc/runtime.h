@@ -28,7 +28,8 @@ int hip_fail(hipError_t e, const char* what);
 
 // Budgeted device-memory pool. Blocks are cached by exact size and reused;
 // `budget` bounds live + cached bytes (cached blocks are released first when
-// a new allocation would exceed it).
+// a new allocation would exceed it). Thread-safe: a proof server's scheduler
+// thread and the caller's thread may share one context's pool.
 class DevicePool {
  public:
   explicit DevicePool(uint64_t budget) : budget_(budget) {}
@@ -36,11 +37,19 @@ class DevicePool {
   int alloc(size_t bytes, void** out);
   void release(void* p);
   void trim();
-  uint64_t held() const { return live_ + cached_; }
-  uint64_t live() const { return live_; }
+  uint64_t held() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return live_ + cached_;
+  }
+  uint64_t live() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return live_;
+  }
   uint64_t budget() const { return budget_; }
 
  private:
+  void trim_locked();
+  mutable std::mutex mu_;
   uint64_t budget_;
   uint64_t live_ = 0, cached_ = 0;
   std::multimap<size_t, void*> free_;
@@ -49,6 +58,7 @@ class DevicePool {
 
 // Pinned host staging blocks, cached by size (uploads/downloads of the small
 // per-proof control and output regions go through one copy each).
+// Thread-safe, as DevicePool.
 class HostPool {
  public:
   ~HostPool();
@@ -56,6 +66,7 @@ class HostPool {
   void release(void* p);
 
  private:
+  std::mutex mu_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> sizes_;
 };
@@ -137,6 +148,7 @@ class Instance {
   virtual void free_buffers() = 0;
   uint32_t rounds = 0;
   uint32_t npts = 0;
+  uint32_t mlen = 0;  // message elements on the wire
   uint32_t chal_bytes = 0;
   uint32_t k = 0;
   uint64_t device_bytes = 0;
@@ -198,23 +210,67 @@ Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, u
                         bool eq);
 ServeKind* make_serve_kind(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
                            bool eq);
-ServeKind* make_serve_kind_bb(uint32_t k, uint32_t npts, bool eq);
-ServeKind* make_serve_kind_bb_bb4(uint32_t k, uint32_t npts, bool eq);
-ServeKind* make_serve_kind_bb4(uint32_t k, uint32_t npts, bool eq);
-ServeKind* make_serve_kind_gl(uint32_t k, uint32_t npts, bool eq);
 BatchGroup* make_batch_group(uint32_t table_field, uint32_t chal_field, uint32_t k,
                              uint32_t npts, bool eq);
-BatchGroup* make_batch_group_bb(uint32_t k, uint32_t npts, bool eq);
-BatchGroup* make_batch_group_bb_bb4(uint32_t k, uint32_t npts, bool eq);
-BatchGroup* make_batch_group_bb4(uint32_t k, uint32_t npts, bool eq);
-BatchGroup* make_batch_group_gl(uint32_t k, uint32_t npts, bool eq);
-// Per-field-combination factories (one translation unit each).
-Instance* make_instance_bb(uint32_t k, uint32_t npts, bool eq);
-Instance* make_instance_bb_bb4(uint32_t k, uint32_t npts, bool eq);
-Instance* make_instance_bb4(uint32_t k, uint32_t npts, bool eq);
-Instance* make_instance_gl(uint32_t k, uint32_t npts, bool eq);
+// Per field pair and factor count (one translation unit each, units.h).
+#define ZS_UNIT_DECL(NAME, KK)                                            \
+  Instance* make_instance_##NAME##_k##KK(uint32_t npts, bool eq);         \
+  BatchGroup* make_batch_group_##NAME##_k##KK(uint32_t npts, bool eq);    \
+  ServeKind* make_serve_kind_##NAME##_k##KK(uint32_t npts, bool eq);
+#define ZS_UNIT_DECL4(NAME) \
+  ZS_UNIT_DECL(NAME, 1) ZS_UNIT_DECL(NAME, 2) ZS_UNIT_DECL(NAME, 3) ZS_UNIT_DECL(NAME, 4)
+ZS_UNIT_DECL4(bb)
+ZS_UNIT_DECL4(bb_bb4)
+ZS_UNIT_DECL4(bb4)
+ZS_UNIT_DECL4(gl)
+#undef ZS_UNIT_DECL4
+#undef ZS_UNIT_DECL
+
+// Message width of a wire form for a round polynomial of npts natural values
+// (0 with an error set when the form does not apply). `product_two`: a
+// two-factor product round (the compressed form's only case).
+inline uint32_t wire_width(const zs_wire* w, uint32_t npts, bool product_two) {
+  if (!w || w->kind == ZS_WIRE_VALUES) return npts;
+  switch (w->kind) {
+    case ZS_WIRE_DOMAIN: {
+      const uint32_t m = w->num_nodes + (w->has_inf ? 1u : 0u);
+      if (m == 0) {
+        set_error("an evaluation domain needs at least one sample point");
+        return 0;
+      }
+      if (m > npts) {
+        set_error("the evaluation domain has " + std::to_string(m) +
+                  " samples; a round polynomial of degree " + std::to_string(npts - 1) +
+                  " is sent with at most " + std::to_string(npts));
+        return 0;
+      }
+      if (w->has_inf && w->inf_pos > w->num_nodes) {
+        set_error("inf_pos must be at most the number of finite nodes");
+        return 0;
+      }
+      if (w->num_nodes && !w->nodes) {
+        set_error("null evaluation-domain nodes");
+        return 0;
+      }
+      return m;
+    }
+    case ZS_WIRE_COEFFS:
+      return npts;
+    case ZS_WIRE_COMPRESSED:
+      if (!product_two) {
+        set_error("compressed product round takes exactly 2 factors");
+        return 0;
+      }
+      return 2;
+  }
+  set_error("unknown wire form");
+  return 0;
+}
 
 uint32_t field_bytes(uint32_t field);
-int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts);
+// Structural checks of a request: the natural value count npts and the
+// message width mlen on its wire form.
+int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts,
+                  uint32_t* mlen = nullptr);
 
 }  // namespace zs

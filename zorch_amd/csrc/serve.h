@@ -104,7 +104,11 @@ class ServeKindImpl final : public ServeKind {
     memset(h, 0, r->rec_bytes);
     Ctl<FC>& c = *(Ctl<FC>*)h;
     init_ctl_consts<FC, K, NPTS>(c);
-    set_ctl_proof<FC>(c, t, a.claim);
+    if ((rc = set_ctl_proof<FC>(c, t, a, NPTS))) {
+      ctx->pool->release(r->block);
+      ctx->host_pool->release(r->host);
+      return rc;
+    }
     if (EQ) fill_eq_point<FC>((TC*)(h + o_eqp(n)), (TC*)(h + o_eqab(n)), a.eq_point, n);
     StageArgs<FT, FC>& sa = r->sa;
     sa.in = stage_in ? (const TT*)(b + o_in) : (const TT*)a.tables;
@@ -144,7 +148,10 @@ class ServeKindImpl final : public ServeKind {
     if (m == 0) return ZS_OK;
     const TuneParams& tune = ctx->tune;
     const uint32_t maxg = tune.max_grid < (uint32_t)MAXG ? tune.max_grid : (uint32_t)MAXG;
-    std::vector<BatchItem> tails, passes;
+    // Pass items by round type (round 0, round 1, later rounds), one launch
+    // each: a launch's kernel is specialised to its round type. Tail items
+    // ride with the later rounds.
+    std::vector<BatchItem> tails, passes[3];
     for (size_t i = 0; i < m; ++i) {
       ServeReq* r = reqs[i];
       const uint32_t round = r->next;
@@ -162,13 +169,14 @@ class ServeKindImpl final : public ServeKind {
         if (grid > maxg) grid = maxg;
         if (grid < 1) grid = 1;
         while (grid & (grid - 1)) grid &= grid - 1;
-        passes.push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, round});
+        passes[round < 2 ? round : 2].push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, round});
         r->next = round + 1;
       }
       r->finishing = r->next == r->n;
     }
-    std::stable_sort(passes.begin(), passes.end(),
-                     [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; });
+    for (auto& list : passes)
+      std::stable_sort(list.begin(), list.end(),
+                       [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; });
     const size_t args_bytes = align256(m * sizeof(StageArgs<FT, FC>));
     const size_t need = args_bytes + m * sizeof(BatchItem);
     // The next staging slot of the ring, free once the step that last used
@@ -193,18 +201,37 @@ class ServeKindImpl final : public ServeKind {
     StageArgs<FT, FC>* args = (StageArgs<FT, FC>*)h;
     for (size_t i = 0; i < m; ++i) args[i] = static_cast<Req*>(reqs[i])->sa;
     BatchItem* items = (BatchItem*)(h + args_bytes);
-    uint32_t off = 0, nitems = 0;
-    for (auto* list : {&tails, &passes}) {
-      for (BatchItem it : *list) {
-        it.block_off = off;
-        off += it.blocks;
-        items[nitems++] = it;
+    // Launch t takes items [first[t], first[t] + count[t]) over blocks[t] blocks.
+    uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0}, blocks[3] = {0, 0, 0};
+    uint32_t nitems = 0;
+    for (int t = 0; t < 3; ++t) {
+      first[t] = nitems;
+      uint32_t off = 0;
+      std::vector<BatchItem>* lists[2] = {&passes[t], t == 2 ? &tails : nullptr};
+      for (auto* list : lists) {
+        if (!list) continue;
+        for (BatchItem it : *list) {
+          it.block_off = off;
+          off += it.blocks;
+          items[nitems++] = it;
+        }
       }
+      count[t] = nitems - first[t];
+      blocks[t] = off;
     }
     ZS_HIP(hipMemcpyAsync(st.dev, h, need, hipMemcpyHostToDevice, s));
     uint8_t* d = (uint8_t*)st.dev;
-    batch_kernel<FT, FC, K, EQ, 3><<<off, PASS_THREADS, 0, s>>>(
-        (const StageArgs<FT, FC>*)d, (const BatchItem*)(d + args_bytes), nitems);
+    const StageArgs<FT, FC>* dargs = (const StageArgs<FT, FC>*)d;
+    const BatchItem* ditems = (const BatchItem*)(d + args_bytes);
+    if (count[0])
+      batch_kernel<FT, FC, K, EQ, 0><<<blocks[0], PASS_THREADS, 0, s>>>(dargs, ditems + first[0],
+                                                                        count[0]);
+    if (count[1])
+      batch_kernel<FT, FC, K, EQ, 1><<<blocks[1], PASS_THREADS, 0, s>>>(dargs, ditems + first[1],
+                                                                        count[1]);
+    if (count[2])
+      batch_kernel<FT, FC, K, EQ, 2><<<blocks[2], PASS_THREADS, 0, s>>>(dargs, ditems + first[2],
+                                                                        count[2]);
     ZS_HIP(hipGetLastError());
     ZS_HIP(hipEventRecord(st.ev, s));
     return ZS_OK;
@@ -221,7 +248,8 @@ class ServeKindImpl final : public ServeKind {
     const uint8_t* h = (const uint8_t*)r->host;
     const Ctl<FC>& c = *(const Ctl<FC>*)h;
     const uint32_t n = r->n;
-    out->msgs.assign(h + o_msgs(), h + o_msgs() + (size_t)n * NPTS * sizeof(TC));
+    out->msgs.resize((size_t)n * c.mlen * sizeof(TC));
+    copy_msg_rows(out->msgs.data(), h + o_msgs(), n, NPTS, c.mlen, sizeof(TC));
     out->point.assign(h + o_point(n), h + o_point(n) + (size_t)n * sizeof(TC));
     out->value.assign((const uint8_t*)&c.claim, (const uint8_t*)&c.claim + sizeof(TC));
     out->final_tables.assign(h + o_final(n), h + o_final(n) + (K + (EQ ? 1 : 0)) * sizeof(TC));
@@ -250,14 +278,11 @@ class ServeKindImpl final : public ServeKind {
   int ring_next_ = 0;
 };
 
-template <class FT, class FC>
-ServeKind* make_typed_serve(uint32_t k, uint32_t npts, bool eq) {
-#define ZS_CASE(KK, EQQ)                                                    \
-  if (k == KK && eq == EQQ && npts == (uint32_t)(KK + 1 + (EQQ ? 1 : 0))) \
-    return new ServeKindImpl<FT, FC, KK, KK + 1 + (EQQ ? 1 : 0), EQQ>();
-  ZS_CASE(1, false) ZS_CASE(2, false) ZS_CASE(3, false) ZS_CASE(4, false)
-  ZS_CASE(1, true) ZS_CASE(2, true) ZS_CASE(3, true) ZS_CASE(4, true)
-#undef ZS_CASE
+// The ServeKindImpl of K factor tables for a message of npts values (K + 1, or K + 2 with the eq factor).
+template <class FT, class FC, int K>
+ServeKind* make_typed_serve_k(uint32_t npts, bool eq) {
+  if (eq && npts == (uint32_t)(K + 2)) return new ServeKindImpl<FT, FC, K, K + 2, true>();
+  if (!eq && npts == (uint32_t)(K + 1)) return new ServeKindImpl<FT, FC, K, K + 1, false>();
   return nullptr;
 }
 

@@ -37,11 +37,14 @@ struct Pending {
   uint64_t id = 0;
   zs_prove_args a{};
   zs_transcript t{};
-  std::vector<uint8_t> claim, eq_point;
+  std::vector<uint8_t> claim, eq_point, eq_mass;
+  bool has_wire = false;
+  zs_wire wire{};              // copied wire form; its nodes in `nodes`
+  std::vector<uint8_t> nodes;
   Clock::time_point t_submit;
   ServeKind* kind = nullptr;
   uint64_t bytes = 0;
-  uint32_t npts = 0;
+  uint32_t npts = 0, mlen = 0;
 };
 
 struct Finished {
@@ -77,7 +80,7 @@ struct Direct {
   hipEvent_t ev = nullptr;
   uint64_t bytes = 0;
   Clock::time_point t_submit;
-  uint32_t n = 0, npts = 0, k = 0, cb = 0;
+  uint32_t n = 0, npts = 0, mlen = 0, k = 0, cb = 0;
   bool eq = false;
 };
 constexpr uint32_t DIRECT_MIN_ROUNDS = 20;
@@ -175,6 +178,7 @@ void direct_start(zs_server* S, Pending& p, std::vector<std::pair<uint64_t, Fini
   d.t_submit = p.t_submit;
   d.n = p.a.rounds;
   d.npts = p.npts;
+  d.mlen = p.mlen;
   d.k = p.a.num_tables;
   d.cb = field_bytes(p.a.chal_field);
   d.eq = eq;
@@ -236,7 +240,7 @@ void collect_directs(zs_server* S) {
       f.err = zs_last_error();
     } else {
       ServeResult& r = f.res;
-      r.msgs.resize((size_t)d.n * d.npts * d.cb);
+      r.msgs.resize((size_t)d.n * d.mlen * d.cb);
       r.point.resize((size_t)d.n * d.cb);
       r.value.resize(d.cb);
       r.final_tables.resize((size_t)(d.k + (d.eq ? 1 : 0)) * d.cb);
@@ -331,6 +335,9 @@ void server_loop(zs_server* S) {
     for (Pending& p : admit) {
       p.a.claim = p.claim.data();
       p.a.eq_point = p.eq_point.empty() ? nullptr : p.eq_point.data();
+      p.a.eq_mass = p.eq_mass.empty() ? nullptr : p.eq_mass.data();
+      p.wire.nodes = p.nodes.empty() ? nullptr : p.nodes.data();
+      p.a.wire = p.has_wire ? &p.wire : nullptr;
       if (p.a.rounds >= DIRECT_MIN_ROUNDS && !S->lanes.empty()) {
         direct_start(S, p, done, freed);
         continue;
@@ -492,17 +499,28 @@ int zs_server_create(zs_ctx* ctx, uint64_t budget, zs_server** out) {
 
 int zs_server_submit(zs_server* S, const zs_prove_args* args, const zs_transcript* t, uint64_t* id) {
   if (!S || !args || !t || !id) return fail(ZS_EINVAL, "null argument");
-  uint32_t npts = 0;
-  int rc = validate_args(*args, t, &npts);
+  uint32_t npts = 0, mlen = 0;
+  int rc = validate_args(*args, t, &npts, &mlen);
   if (rc) return rc;
   const bool eq = args->mode == ZS_MODE_EQ;
   const uint32_t cb = field_bytes(args->chal_field);
   Pending p;
   p.a = *args;
+  p.a.wire = nullptr;  // re-pointed at the copy on admission
   p.t = *t;
+  p.mlen = mlen;
+  if (args->wire) {
+    p.has_wire = true;
+    p.wire = *args->wire;
+    if (p.wire.kind == ZS_WIRE_DOMAIN && p.wire.num_nodes)
+      p.nodes.assign((const uint8_t*)p.wire.nodes,
+                     (const uint8_t*)p.wire.nodes + (size_t)p.wire.num_nodes * cb);
+  }
   p.claim.assign((const uint8_t*)args->claim, (const uint8_t*)args->claim + cb);
   if (eq) p.eq_point.assign((const uint8_t*)args->eq_point,
                             (const uint8_t*)args->eq_point + (size_t)args->rounds * cb);
+  if (eq && args->eq_mass)
+    p.eq_mass.assign((const uint8_t*)args->eq_mass, (const uint8_t*)args->eq_mass + cb);
   p.t_submit = Clock::now();
   const KindKey key{args->table_field, args->chal_field, args->num_tables, npts, eq};
   std::lock_guard<std::mutex> lk(S->mu);

@@ -24,13 +24,15 @@ class RunningClaim:
 
 
 class ProverRound(Protocol):
-    """One prover recurrence step: fold the carry and emit a proof message."""
+    """A prover step under fold_rounds: (carry, transcript) -> (carry,
+    transcript, message) (reference zorch/round.py:52-60)."""
 
     def __call__(self, carry: Any, transcript: Any, /) -> tuple[Any, Any, Any]: ...
 
 
 class VerifierRound(Protocol):
-    """The dual step: consume the message, fold the carry, report a verdict."""
+    """A verifier step: (carry, transcript, message) -> (carry, transcript,
+    ok); whatever the step derives travels in the carry (zorch/round.py:63-78)."""
 
     def __call__(self, carry: Any, transcript: Any, message: Any, /) -> tuple[Any, Any, Any]: ...
 

@@ -169,6 +169,26 @@ def el_buffer(value: FieldArray, field: Field):
     return np.ascontiguousarray(value.astype(field).words)
 
 
+def wire_struct(kind: int, domain: Any, chal_field: Any):
+    """(``zs_wire`` or None, keepalive) for a message wire form: ZS_WIRE_VALUES
+    (None), ZS_WIRE_DOMAIN with an EvalDomain, ZS_WIRE_COEFFS or
+    ZS_WIRE_COMPRESSED."""
+    if kind == _lib.ZS_WIRE_VALUES:
+        return None, None
+    if kind == _lib.ZS_WIRE_DOMAIN:
+        return domain.wire(chal_field)
+    return _lib.WireC(kind=kind), None
+
+
+def message_width(args: "_lib.ProveArgsC", transcript=None) -> int:
+    """Elements per round message of a request (``zs_prove_check``); raises
+    ValueError on a malformed request."""
+    out = ctypes.c_uint32(0)
+    st = ctypes.byref(transcript) if transcript is not None else None
+    _lib.check(_lib.lib().zs_prove_check(ctypes.byref(args), st, ctypes.byref(out)))
+    return int(out.value)
+
+
 class DeviceFoldState:
     """Device-resident folding state of one sumcheck (a ``zs_state``).
 
@@ -177,7 +197,9 @@ class DeviceFoldState:
     """
 
     def __init__(self, tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
-                 eq_point: FieldArray | None = None, ctx: Context | None = None):
+                 eq_point: FieldArray | None = None, ctx: Context | None = None, *,
+                 wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None,
+                 eq_mass: FieldArray | None = None):
         self.ctx = ctx or default_context()
         ptr, on_dev, keep, shape = tables_pointer(tables)
         self.k, width = shape
@@ -189,12 +211,17 @@ class DeviceFoldState:
         self.npts = degree + 1 + (1 if mode == _lib.ZS_MODE_EQ else 0)
         claim_buf = el_buffer(claim, self.chal_field)
         eq_buf = el_buffer(eq_point, self.chal_field) if eq_point is not None else None
+        wire, wire_keep = wire_struct(wire_kind, domain, self.chal_field)
+        mass_buf = el_buffer(eq_mass, self.chal_field) if eq_mass is not None else None
         args = _lib.ProveArgsC(
             table_field=self.table_field.fid, chal_field=self.chal_field.fid, mode=mode,
             degree=degree, num_tables=self.k, rounds=self.rounds, tables=ptr,
             tables_on_device=1 if on_dev else 0, reserved=0,
             eq_point=eq_buf.ctypes.data if eq_buf is not None else None,
-            claim=claim_buf.ctypes.data)
+            claim=claim_buf.ctypes.data,
+            wire=ctypes.pointer(wire) if wire is not None else None,
+            eq_mass=mass_buf.ctypes.data if mass_buf is not None else None)
+        self.mlen = message_width(args)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().zs_state_create(self.ctx.handle, ctypes.byref(args), ctypes.byref(h)))
         self._h = h
@@ -205,7 +232,7 @@ class DeviceFoldState:
     def round(self, transcript):
         """One round: returns (msg, raw challenge, reduced claim, new transcript)."""
         f = self.chal_field
-        msg = np.zeros((self.npts, f.words), dtype=f.word_dtype)
+        msg = np.zeros((self.mlen, f.words), dtype=f.word_dtype)
         chal = np.zeros((f.words,), dtype=f.word_dtype)
         claim = np.zeros((f.words,), dtype=f.word_dtype)
         st = transcript.state.copy()

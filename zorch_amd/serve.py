@@ -17,13 +17,19 @@ from typing import Any
 
 from zorch_amd import _lib
 from zorch_amd.fields import FieldArray
-from zorch_amd.runtime import Context, default_context
+from zorch_amd.runtime import Context
 from zorch_amd.sumcheck._device import DeviceProof, _result, make_job
 
 
 class ProofServer:
+    """A proof server on a context of its own (``ctx=None``, the default) or on
+    `ctx`, which it takes over: the server's scheduler thread owns the
+    context's stream and launch shapes until ``close``. Inputs resident on
+    another context of the same device may be submitted."""
+
     def __init__(self, ctx: Context | None = None, hbm_budget_bytes: int = 0):
-        self.ctx = ctx or default_context()
+        self._own_ctx = ctx is None
+        self.ctx = ctx if ctx is not None else Context(0, hbm_budget_bytes)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().zs_server_create(self.ctx.handle, hbm_budget_bytes,
                                                ctypes.byref(h)))
@@ -102,6 +108,9 @@ class ProofServer:
             _lib.lib().zs_server_destroy(self._h)
             self._h = None
             self._jobs.clear()
+        if self._own_ctx and self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
 
     def __enter__(self) -> "ProofServer":
         return self

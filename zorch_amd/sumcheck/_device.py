@@ -16,7 +16,8 @@ import numpy as np
 
 from zorch_amd import _lib
 from zorch_amd.fields import Field, FieldArray, as_field
-from zorch_amd.runtime import Context, default_context, el_buffer, log2_strict, tables_pointer
+from zorch_amd.runtime import (Context, default_context, el_buffer, log2_strict, message_width,
+                               tables_pointer, wire_struct)
 
 
 @dataclass
@@ -47,29 +48,35 @@ class StageJob:
 
 
 def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
-             eq_point: FieldArray | None = None) -> StageJob:
+             eq_point: FieldArray | None = None, *, wire_kind: int = _lib.ZS_WIRE_VALUES,
+             domain: Any = None) -> StageJob:
+    """Arguments and output buffers of one stage; `wire_kind` / `domain` pick
+    the message wire form (natural values by default)."""
     chal = as_field(chal_field)
     ptr, on_dev, keep, shape = tables_pointer(tables)
     k, width = shape
     n = log2_strict(width)
     table_field = tables.dtype if not isinstance(tables, FieldArray) else tables.field
-    npts = degree + 1 + (1 if mode == _lib.ZS_MODE_EQ else 0)
     claim_buf = el_buffer(claim, chal)
     eq_buf = el_buffer(eq_point, chal) if eq_point is not None else None
     if mode == _lib.ZS_MODE_EQ and (eq_point is None or eq_point.shape[0] != n):
         raise ValueError(f"equality point needs {n} coordinates")
+    wire, wire_keep = wire_struct(wire_kind, domain, chal)
+    wire_ptr = ctypes.pointer(wire) if wire is not None else None
     args = _lib.ProveArgsC(
         table_field=as_field(table_field).fid, chal_field=chal.fid, mode=mode, degree=degree,
         num_tables=k, rounds=n, tables=ptr, tables_on_device=1 if on_dev else 0, reserved=0,
-        eq_point=eq_buf.ctypes.data if eq_buf is not None else None, claim=claim_buf.ctypes.data)
+        eq_point=eq_buf.ctypes.data if eq_buf is not None else None, claim=claim_buf.ctypes.data,
+        wire=wire_ptr)
+    npts = message_width(args)
     msgs = np.zeros((n, npts, chal.words), dtype=chal.word_dtype)
     point = np.zeros((n, chal.words), dtype=chal.word_dtype)
     value = np.zeros((chal.words,), dtype=chal.word_dtype)
     final = np.zeros((k + 1, chal.words), dtype=chal.word_dtype)
     proof = _lib.ProofC(msgs=msgs.ctypes.data, point=point.ctypes.data, value=value.ctypes.data,
                         final_tables=final.ctypes.data)
-    return StageJob(args, proof, [keep, claim_buf, eq_buf], chal, npts, k, n, mode, msgs, point,
-                    value, final)
+    return StageJob(args, proof, [keep, claim_buf, eq_buf, wire, wire_ptr, wire_keep], chal,
+                    npts, k, n, mode, msgs, point, value, final)
 
 
 def _result(job: StageJob, st: _lib.TranscriptC, transcript) -> DeviceProof:
@@ -80,9 +87,11 @@ def _result(job: StageJob, st: _lib.TranscriptC, transcript) -> DeviceProof:
 
 
 def prove_stage(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int, transcript,
-                eq_point: FieldArray | None = None, ctx: Context | None = None) -> DeviceProof:
+                eq_point: FieldArray | None = None, ctx: Context | None = None, *,
+                wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None) -> DeviceProof:
     ctx = ctx or default_context()
-    job = make_job(tables, claim, chal_field, mode, degree, eq_point)
+    job = make_job(tables, claim, chal_field, mode, degree, eq_point, wire_kind=wire_kind,
+                   domain=domain)
     st = transcript.state.copy()
     _lib.check(_lib.lib().zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
                                    ctypes.byref(job.proof)))

@@ -23,7 +23,8 @@ from zorch_amd.verify import verify
 
 @dataclass(frozen=True)
 class EqSumClaim:
-    """Public sum claim for ``eq(equality_point, x) * summand(x)``."""
+    """Stated total of eq(equality_point, x) times the summand over the cube
+    (reference zorch/sumcheck/eq/stage.py:24-30)."""
 
     equality_point: FieldArray
     value: FieldArray
@@ -32,7 +33,8 @@ class EqSumClaim:
 
 @dataclass(frozen=True)
 class EqPolyWitness:
-    """Factor tables witnessing an ``EqSumClaim``."""
+    """The stacked summand factors of an ``EqSumClaim`` (the eq factor is not
+    among them: it is derived from the claim's point)."""
 
     factors: object
 
@@ -44,8 +46,8 @@ def _check_claim(claim: EqSumClaim) -> None:
 
 
 class EqPolyProver:
-    """Prove equality-weighted sumcheck with the equality factor separate
-    (zorch/sumcheck/eq/stage.py:48-92)."""
+    """Prover role of the eq-weighted sumcheck: eq(w, x) stays a factored
+    weight beside the summand tables (zorch/sumcheck/eq/stage.py:48-92)."""
 
     def __init__(self, summand: ProductSummand, *, challenges) -> None:
         if type(summand) is not ProductSummand:

@@ -1,11 +1,14 @@
 """Sumcheck prover rounds (drop-in for reference zorch/sumcheck/prover.py).
 
 ``StandardRound(ProductSummand(d), challenges=C)`` is the reference's plain
-materialised product round on the natural domain (prover.py:127-174): send
-s(t) = sum_j prod_k (lo_k[j] + t (hi_k[j] - lo_k[j])) for t = 0..d, draw the
-challenge, reduce the claim, fold MSB-first. Here the tables live on the GPU
-(``DeviceFoldState``) and each call is one fused fold+evaluate pass plus the
-device Fiat-Shamir hop (``zs_state_round``).
+materialised product round (prover.py:127-174): send s(t) =
+sum_j prod_k (lo_k[j] + t (hi_k[j] - lo_k[j])) sampled on its domain (the
+naturals t = 0..d by default), draw the challenge, reduce the claim, fold
+MSB-first. ``CompressedProductRound`` sends [c_0, c_2] (prover.py:177-212)
+and ``CoeffsProductRound`` the ascending coefficients. Here the tables live on
+the GPU (``DeviceFoldState``) and each call is one fused fold+evaluate pass
+plus the device Fiat-Shamir hop (``zs_state_round``); the message form is a
+map the device finisher applies to the round's natural values.
 
 Messages, claims and points are in the challenge field; base-field tables are
 lifted into it (the reference's tested pattern for extension challenges,
@@ -66,11 +69,17 @@ class ProductSummand:
         return self.combine(self.combine_scalars(), *factors)
 
 
-def _check_domain(domain: EvalDomain | None, degree: int) -> None:
+def round_wire(domain: EvalDomain | None, npts: int) -> tuple[int, EvalDomain | None]:
+    """The device wire form of messages sampled on `domain` for a round
+    polynomial of npts natural values: the natural domain itself (or None) is
+    the plain values; any other domain is sampled from them on the device."""
     if domain is None:
-        return
-    if not domain.is_natural() or domain.nodes.shape[0] != degree + 1:
-        raise ValueError("only the natural domain {0..degree} is supported by the device prover")
+        return _lib.ZS_WIRE_VALUES, None
+    if domain.nodes is None:
+        raise ValueError("a sampling domain needs an explicit node set")
+    if domain.is_natural() and domain.nodes.shape[0] == npts:
+        return _lib.ZS_WIRE_VALUES, None
+    return _lib.ZS_WIRE_DOMAIN, domain
 
 
 def _advance_device(carry: FoldingClaim, transcript, make_state) -> tuple:
@@ -84,35 +93,92 @@ def _advance_device(carry: FoldingClaim, transcript, make_state) -> tuple:
     return FoldingClaim(state, carry.claim.bind(reduced, chal)), transcript, msg
 
 
-class StandardRound:
-    """The plain materialised product round on the natural domain."""
+def _scratch_message(make_state, tables, chal_dtype) -> FieldArray:
+    """A round's message for stacked tables without advancing any caller
+    state: a scratch device state and a scratch transcript absorb the hop."""
+    from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
 
-    def __init__(self, summand: ProductSummand, domain: EvalDomain | None = None, *,
-                 challenges: ChallengePolicy):
-        _check_domain(domain, summand.degree)
-        self.summand = summand
-        self.domain = domain
-        self.challenges = challenges
+    st = make_state(tables, zeros((), chal_dtype))
+    msg, _, _, _ = st.round(Sha256FieldTranscript.new(b"", chal_dtype.base))
+    return msg
+
+
+class _DeviceProductRound:
+    """Shared device plumbing of the product rounds: the wire form decides the
+    message; tables fold MSB-first on the device (prover.py:76-83)."""
+
+    summand: ProductSummand
+    challenges: ChallengePolicy
+
+    def _stage_wire(self) -> tuple[int, EvalDomain | None]:
+        raise NotImplementedError
 
     def _make_state(self, tables, claim: FieldArray) -> DeviceFoldState:
         if not isinstance(tables, (FieldArray, DeviceTables)):
             raise TypeError("state must be stacked tables (FieldArray or DeviceTables)")
+        kind, domain = self._stage_wire()
         return DeviceFoldState(tables, claim, self.challenges.dtype, _lib.ZS_MODE_PRODUCT,
-                               self.summand.degree)
+                               self.summand.degree, wire_kind=kind, domain=domain)
 
     def _round_poly(self, folded) -> FieldArray:
-        """s sampled on {0..degree} for stacked tables, without advancing any
-        caller state (a scratch transcript absorbs the hop)."""
-        from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
-
-        claim = zeros((), self.challenges.dtype)
-        st = self._make_state(folded, claim)
-        msg, _, _, _ = st.round(Sha256FieldTranscript.new(b"", self.challenges.dtype.base))
-        return msg
+        """The round message for stacked tables, without advancing any caller
+        state (a scratch transcript absorbs the hop)."""
+        return _scratch_message(self._make_state, folded, self.challenges.dtype)
 
     def __call__(self, carry: FoldingClaim, transcript):
         return _advance_device(carry, transcript,
                                lambda tables: self._make_state(tables, carry.claim.value))
+
+
+class StandardRound(_DeviceProductRound):
+    """The plain materialised product round (prover.py:127-174): messages
+    sampled on `domain` (the natural {0..degree} when None) and reduced by
+    Lagrange evaluation, or by the domain's value-to-coefficient map
+    (reduce_domain) for an explicit domain."""
+
+    def __init__(self, summand: ProductSummand, domain: EvalDomain | None = None, *,
+                 challenges: ChallengePolicy):
+        self.summand = summand
+        self.domain = domain
+        self.challenges = challenges
+        round_wire(domain, summand.degree + 1)  # rejects a node-less domain early
+
+    def _stage_wire(self):
+        return round_wire(self.domain, self.summand.degree + 1)
+
+
+class CompressedProductRound(_DeviceProductRound):
+    """Two-factor product round on the compressed coefficient wire [c_0, c_2]
+    (prover.py:177-212): c_0 = sum f0 b0, c_2 = sum (f1 - f0)(b1 - b0); the
+    linear coefficient stays off the wire and the claim reduces through
+    s(0) + s(1) = claim (reduce_compressed). Split and fold are those of
+    ``StandardRound(ProductSummand(2))``."""
+
+    def __init__(self, challenges: ChallengePolicy) -> None:
+        self.challenges = challenges
+        self.summand = ProductSummand(2)
+
+    def _make_state(self, tables, claim: FieldArray) -> DeviceFoldState:
+        if hasattr(tables, "shape") and tables.shape[0] != 2:
+            raise ValueError(f"compressed product round takes exactly 2 factors, got "
+                             f"{tables.shape[0]}")
+        return super()._make_state(tables, claim)
+
+    def _stage_wire(self):
+        return _lib.ZS_WIRE_COMPRESSED, None
+
+
+class CoeffsProductRound(_DeviceProductRound):
+    """Product round sending the ascending coefficients of its round
+    polynomial (the wire ``CoeffsSumcheckRound`` checks; the message of
+    ``product_round_coeffs``, domain.py:216-223), reduced by reduce_coeffs."""
+
+    def __init__(self, summand: ProductSummand, *, challenges: ChallengePolicy) -> None:
+        self.summand = summand
+        self.challenges = challenges
+
+    def _stage_wire(self):
+        return _lib.ZS_WIRE_COEFFS, None
 
 
 def product(factors: Sequence[FieldArray]) -> FieldArray:
@@ -120,5 +186,6 @@ def product(factors: Sequence[FieldArray]) -> FieldArray:
 
 
 __all__ = [
-    "FoldingClaim", "initial_claim", "ProductSummand", "StandardRound", "product", "fsum",
+    "FoldingClaim", "initial_claim", "ProductSummand", "StandardRound", "CompressedProductRound",
+    "CoeffsProductRound", "round_wire", "product", "fsum",
 ]

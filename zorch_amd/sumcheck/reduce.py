@@ -1,5 +1,9 @@
-"""Per-round claim reductions (reference zorch/sumcheck/reduce.py:18-52) and
-univariate helpers (zorch/poly/univariate.py:36-65), on host field arrays.
+"""Per-round claim reductions (reference zorch/sumcheck/reduce.py:18-66) and
+univariate helpers (zorch/poly/univariate.py:36-131), on host field arrays.
+
+One reduction per wire form, each returning ``(reduced, ok)`` with ``ok`` the
+round's own consistency check; both roles use them (the device prover applies
+the same maps in its finisher).
 """
 
 from __future__ import annotations
@@ -36,7 +40,37 @@ def eval_univariate(evals: FieldArray, x: FieldArray) -> FieldArray:
     return fsum(evals * basis)
 
 
+def eval_coeffs(coeffs: FieldArray, x: FieldArray) -> FieldArray:
+    """sum_i coeffs[i] x^i by Horner (univariate.py:109-131)."""
+    out = coeffs[coeffs.shape[0] - 1]
+    for i in range(coeffs.shape[0] - 2, -1, -1):
+        out = out * x + coeffs[i]
+    return out
+
+
 def reduce_evals(claim: FieldArray, msg: FieldArray, r: FieldArray, degree: int):
     """`msg` is s on the naturals {0..degree}: (s(r), claim == s(0) + s(1))."""
     require_width(msg, degree + 1)
     return eval_univariate(msg, r), bool(claim == msg[0] + msg[1])
+
+
+def reduce_coeffs(claim: FieldArray, msg: FieldArray, r: FieldArray, degree: int):
+    """`msg` is s's ascending coefficients: s(0) = c_0 and s(1) = sum c."""
+    require_width(msg, degree + 1, "coefficients")
+    return eval_coeffs(msg, r), bool(claim == msg[0] + fsum(msg))
+
+
+def reduce_domain(claim: FieldArray, msg: FieldArray, r: FieldArray, domain):
+    """`msg` is s sampled at `domain`'s points: its coefficients through the
+    domain's value-to-coefficient map, evaluated at r."""
+    coeffs = domain.to_coeffs(msg)
+    return eval_coeffs(coeffs, r), bool(claim == coeffs[0] + fsum(coeffs))
+
+
+def reduce_compressed(claim: FieldArray, msg: FieldArray, r: FieldArray):
+    """`msg` is [c_0, c_2]; c_1 = claim - 2 c_0 - c_2 spends the
+    s(0) + s(1) = claim identity, so ``ok`` is constant true."""
+    require_width(msg, 2, "coefficients [c_0, c_2]")
+    c0, c2 = msg[0], msg[1]
+    c1 = claim - c0 - c0 - c2
+    return eval_coeffs(stack([c0, c1, c2]), r), True

@@ -1,8 +1,13 @@
-"""Sumcheck verifier round (drop-in for reference zorch/sumcheck/verifier.py:44-71).
+"""Sumcheck verifier rounds (drop-in for reference zorch/sumcheck/verifier.py:45-134).
 
-``SumcheckRound(degree, challenges)`` checks ``claim == s(0) + s(1)`` and
-reduces the claim to ``s(r)`` with the challenge drawn exactly as the prover
-drew it. The step runs in the library's host code (``zs_verify_round``).
+``SumcheckRound(degree, challenges)`` checks ``claim == s(0) + s(1)`` on the
+natural-domain values and reduces the claim to ``s(r)`` with the challenge
+drawn exactly as the prover drew it (the library's host code,
+``zs_verify_round``). ``CoeffsSumcheckRound`` is the same check for ascending
+coefficients (s(0) = c_0, s(1) = sum c), and ``CompressedCoeffsSumcheckRound``
+the compressed [c_0, c_2] wire, whose c_1 is rebuilt from the claim (its ``ok``
+is constant true; binding rests on the terminal check). Those two reduce on
+host field arrays (``sumcheck.reduce``) over the library's host transcript.
 """
 
 from __future__ import annotations
@@ -16,7 +21,8 @@ from zorch_amd import _lib
 from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field
 from zorch_amd.round import RunningClaim
-from zorch_amd.sumcheck.reduce import reduce_evals
+from zorch_amd.sumcheck.reduce import (reduce_coeffs, reduce_compressed, reduce_evals,
+                                       require_width)
 
 
 @dataclass(frozen=True)
@@ -54,3 +60,50 @@ class SumcheckRound:
             chal.ctypes.data, red.ctypes.data, ctypes.byref(ok)))
         new_t = type(transcript)(st, transcript.dtype)
         return claim.bind(FieldArray(red, f), FieldArray(chal, f)), new_t, bool(ok.value)
+
+
+def _message_in(challenges: ChallengePolicy, msg: FieldArray, value: FieldArray):
+    f = as_field(challenges.dtype)
+    if msg.field is not f:
+        raise ValueError("round messages must be in the challenge field "
+                         f"{f.name}, got {msg.field.name}")
+    if msg.ndim != 1:
+        raise ValueError("a round message is one row of the proof")
+    return value.astype(f)
+
+
+@dataclass(frozen=True)
+class CoeffsSumcheckRound:
+    """Verifier of a coefficient-form round (verifier.py:76-100)."""
+
+    degree: int
+    challenges: ChallengePolicy
+
+    def __post_init__(self) -> None:
+        if self.degree < 1:
+            raise ValueError("degree must be >= 1")
+
+    def __call__(self, claim: RunningClaim, transcript, msg: FieldArray):
+        # Structural rejection precedes any read of the claim.
+        require_width(msg, self.degree + 1, "coefficients")
+        value = _message_in(self.challenges, msg, claim.value)
+        transcript, r = self.challenges.observe_and_sample(transcript, msg)
+        reduced, ok = reduce_coeffs(value, msg, r, self.degree)
+        return claim.bind(reduced, r), transcript, ok
+
+
+@dataclass(frozen=True)
+class CompressedCoeffsSumcheckRound:
+    """Verifier of the compressed [c_0, c_2] wire (verifier.py:104-134)."""
+
+    challenges: ChallengePolicy
+
+    def check_reduce(self, claim: FieldArray, msg: FieldArray, r: FieldArray):
+        """Rebuild c_1 from the claim and reduce, for an external challenge."""
+        return reduce_compressed(claim, msg, r)
+
+    def __call__(self, claim: RunningClaim, transcript, msg: FieldArray):
+        value = _message_in(self.challenges, msg, claim.value)
+        transcript, r = self.challenges.observe_and_sample(transcript, msg)
+        reduced, ok = self.check_reduce(value, msg, r)
+        return claim.bind(reduced, r), transcript, ok
