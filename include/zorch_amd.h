@@ -256,6 +256,20 @@ int zs_prove_check(const zs_prove_args* args, const zs_transcript* transcript,
  * the Fiat-Shamir hop on the device; `transcript` is read and advanced. */
 int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
              zs_proof* out);
+/* No counterpart in the reference. For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): zs_prove with the proof's device working set (control block,
+ * messages, slot rows, eq families, folded tables and, for host tables, the
+ * staged input) in a library-owned scratch array of ZS_NOALLOC_SCRATCH_BYTES
+ * per device, held until the process ends; the call allocates no device or
+ * pinned host memory. ZS_EINVAL when the stage does not fit (for instance a
+ * two-table BabyBear^4 zerocheck with device-resident tables fits up to
+ * n = 21). Calls that overlap on one device race on its array: serialise
+ * them. Results are those of zs_prove. */
+#define ZS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int zs_prove_noalloc(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
+                     zs_proof* out);
+
 /* Many independent instances (uneven sizes, mixed kinds) in one submission;
  * instance i uses args[i], transcripts[i], outs[i], with the same results as
  * zs_prove on each. Instances of one kind (field pair, factor count, mode)

@@ -18,6 +18,7 @@ ZS_F_BABYBEAR, ZS_F_BABYBEAR4, ZS_F_GOLDILOCKS = 1, 2, 3
 ZS_TR_SHA256, ZS_TR_BLAKE3 = 1, 2
 ZS_MODE_PRODUCT, ZS_MODE_EQ = 1, 2
 ZS_MAX_ROUNDS = 31
+ZS_NOALLOC_SCRATCH_BYTES = 64 << 20
 ZS_WIRE_VALUES, ZS_WIRE_DOMAIN, ZS_WIRE_COEFFS, ZS_WIRE_COMPRESSED = 0, 1, 2, 3
 ZS_OP_ADD, ZS_OP_SUB, ZS_OP_MUL = 1, 2, 3
 
@@ -172,6 +173,8 @@ SIGNATURES = {
     "zs_field_op": (_INT, [_P, _U32, _U32, _P, _P, _P, _SZ]),
     "zs_prove": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                         ctypes.POINTER(ProofC)]),
+    "zs_prove_noalloc": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
+                                ctypes.POINTER(ProofC)]),
     "zs_prove_batch": (_INT, [_P, _SZ, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                               ctypes.POINTER(ProofC)]),
     "zs_state_create": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(_P)]),

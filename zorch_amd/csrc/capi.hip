@@ -321,13 +321,17 @@ int zs_ctx_create(int device, uint64_t budget, zs_ctx** out) {
   if (e != hipSuccess || count == 0) return fail(ZS_ENODEV, "no HIP device available");
   if (device < 0 || device >= count) return fail(ZS_EINVAL, "device index out of range");
   ZS_HIP(hipSetDevice(device));
-  auto* ctx = new zs_ctx();
-  ctx->device = device;
   if (budget == 0) {
     size_t free_b = 0, total_b = 0;
     ZS_HIP(hipMemGetInfo(&free_b, &total_b));
     budget = (uint64_t)(free_b * 0.9);
   }
+  // The no-allocation entry point's scratch array, looked up once per device
+  // with the device current.
+  int rc = noalloc_scratch_init(device);
+  if (rc) return rc;
+  auto* ctx = new zs_ctx();
+  ctx->device = device;
   ctx->pool = new DevicePool(budget);
   ctx->host_pool = new HostPool();
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

@@ -98,6 +98,12 @@ class InstanceImpl final : public Instance {
   using Instance::setup;
   int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t, hipStream_t s,
             const TuneParams& tune) override {
+    return setup_in(ctx, a, t, s, tune, nullptr, 0, nullptr, 0);
+  }
+
+  int setup_in(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t, hipStream_t s,
+               const TuneParams& tune, void* dev_mem, size_t dev_cap, void* host_mem,
+               size_t host_cap) override {
     ctx_ = ctx;
     n_ = a.rounds;
     rounds = n_;
@@ -132,11 +138,23 @@ class InstanceImpl final : public Instance {
     const bool stage_in = !a.tables_on_device;
     if (stage_in) o_in = take((size_t)K * N * sizeof(TT));
     void* block = nullptr;
-    int rc = ctx->pool->alloc(off, &block);
-    if (rc) return rc;
+    int rc = ZS_OK;
+    if (dev_mem) {
+      // Caller-provided memory: nothing is allocated or released here.
+      if (off > dev_cap)
+        return fail(ZS_EINVAL, "the stage needs " + std::to_string(off) +
+                                   " bytes of device memory; the buffer holds " +
+                                   std::to_string(dev_cap));
+      if (io_bytes_ > host_cap) return fail(ZS_EINVAL, "host staging buffer too small");
+      external_ = true;
+      block = dev_mem;
+      host_ = host_mem;
+    } else {
+      if ((rc = ctx->pool->alloc(off, &block))) return rc;
+    }
     block_ = block;
     device_bytes = off;
-    if ((rc = ctx->host_pool->alloc(io_bytes_, &host_))) return rc;
+    if (!external_ && (rc = ctx->host_pool->alloc(io_bytes_, &host_))) return rc;
     uint8_t* b = (uint8_t*)block;
     args_.in = stage_in ? (const TT*)(b + o_in) : (const TT*)a.tables;
     args_.in_stride = N;
@@ -456,8 +474,8 @@ class InstanceImpl final : public Instance {
   }
 
   void free_buffers() override {
-    if (block_ && ctx_) ctx_->pool->release(block_);
-    if (host_ && ctx_) ctx_->host_pool->release(host_);
+    if (!external_ && block_ && ctx_) ctx_->pool->release(block_);
+    if (!external_ && host_ && ctx_) ctx_->host_pool->release(host_);
     block_ = nullptr;
     host_ = nullptr;
   }
@@ -466,6 +484,7 @@ class InstanceImpl final : public Instance {
   zs_ctx* ctx_ = nullptr;
   uint32_t n_ = 0;
   TuneParams tune_;
+  bool external_ = false;  // block_ and host_ belong to the caller
   void* block_ = nullptr;
   void* host_ = nullptr;
   size_t o_ctl_ = 0, o_msgs_ = 0, o_point_ = 0, o_final_ = 0, out_bytes_ = 0, io_bytes_ = 0;

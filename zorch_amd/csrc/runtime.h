@@ -133,6 +133,12 @@ class Instance {
   int setup(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t) {
     return setup(ctx, a, t, ctx->stream, ctx->tune);
   }
+  // As setup, in caller-provided device memory (dev_cap bytes) and host
+  // staging (host_cap bytes, any host memory): allocates nothing; ZS_EINVAL if
+  // the stage does not fit.
+  virtual int setup_in(zs_ctx* ctx, const zs_prove_args& a, const zs_transcript& t,
+                       hipStream_t stream, const TuneParams& tune, void* dev_mem, size_t dev_cap,
+                       void* host_mem, size_t host_cap) = 0;
   // Enqueue rounds [first, last) on `stream`.
   virtual int enqueue(uint32_t first, uint32_t last, hipStream_t stream) = 0;
   // Enqueue the copy of the outputs to pinned host staging (whole proof).
@@ -266,6 +272,10 @@ inline uint32_t wire_width(const zs_wire* w, uint32_t npts, bool product_two) {
   set_error("unknown wire form");
   return 0;
 }
+
+// Device address of the no-allocation scratch array on `device` (current),
+// looked up once per device (noalloc.hip).
+int noalloc_scratch_init(int device);
 
 uint32_t field_bytes(uint32_t field);
 // Structural checks of a request: the natural value count npts and the
