@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the dense sumcheck prover on one MI355X.
 
-Default workload: the zerocheck-style degree-3 sumcheck eq(z,x) A(x) B(x) at
-n = 24 (BabyBear tables, BabyBear^4 challenges and z, SHA-256 field
-transcript over BabyBear), tables resident in HBM before timing starts. One
-step is one whole proof (24 rounds: fused fold+evaluate passes, device
-Fiat-Shamir hops, claim reductions) from the host call to the returned proof.
+The metric has three parts, all in the one JSON line a default run prints:
 
-Prints one JSON line: table-elems/s (k * 2^n per proof) as ``value``, the
-roofline of the dominant kernel from HIP events on the prover's stream, and
-the CPU baseline (the C oracle on the host cores).
+* the single proof: the zerocheck-style degree-3 sumcheck eq(z,x) A(x) B(x)
+  at n = 24 (BabyBear tables, BabyBear^4 challenges and z, SHA-256 field
+  transcript over BabyBear), tables resident in HBM before timing starts. One
+  step is one whole proof (24 rounds: fused fold+evaluate passes, device
+  Fiat-Shamir hops, claim reductions) from the host call to the returned
+  proof; ``value`` is table-elems/s (k * 2^n per proof), with the roofline of
+  the dominant kernel from HIP events on the prover's stream, the proof's own
+  fraction of the nominal and of the measured HBM peak, and the CPU baseline
+  (the C oracle on the host cores, one thread and all of them);
+* ``batch``: 512 independent instances (n ~ U{12..22}, product or
+  zerocheck) in one zs_prove_batch against 512 one-proof calls;
+* ``serve``: Poisson arrivals at 2000 requests/s over 60 s (zerocheck,
+  n ~ U{14..24}) through the continuous-batching server, p50/p99 latency and
+  proofs/s, against one proof per call.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
+                    [--no-batch] [--no-serve] [--no-cpu-baseline]
 """
 
 from __future__ import annotations
@@ -136,18 +144,20 @@ def _verify_proof(inst, msgs, t0) -> bool:
         SumClaim(inst.claim, inst.n), proof, t0).ok
 
 
-def bench_batch(args) -> dict:
+def bench_batch(args, insts=None) -> dict:
     """512 independent instances (sizes 2^12..2^22, product or zerocheck):
-    one zs_prove_batch call against 512 sequential zs_prove calls."""
+    one zs_prove_batch call against 512 sequential zs_prove calls, on a
+    context of its own (the autotuned shapes cached as "batch")."""
     from zorch_amd import _lib
     from zorch_amd import workloads as W
-    from zorch_amd.runtime import DeviceTables, default_context
+    from zorch_amd.runtime import Context, DeviceTables
     from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
     from zorch_amd.sumcheck._device import make_job
 
     spec = W.batch_spec(count=args.batch_count)
-    insts = [W.batch_instance(spec, i) for i in range(len(spec.sizes))]
-    ctx = default_context()
+    if insts is None:
+        insts = W.batch_instances(spec)
+    ctx = Context(0)
     lib = _lib.lib()
     dev = [DeviceTables.from_host(i.tables, ctx) for i in insts]
     modes = [_lib.ZS_MODE_EQ if i.mode == "zerocheck" else _lib.ZS_MODE_PRODUCT for i in insts]
@@ -167,9 +177,9 @@ def bench_batch(args) -> dict:
             _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(j.args), ctypes.byref(st),
                                     ctypes.byref(j.proof)))
 
+    single_tune = ctx.get_tune()
     plain()
     plain_msgs = [j.msgs.copy() for j in jobs]
-    default_tune = ctx.get_tune()
     batched()
     t = time.perf_counter()
     batched()
@@ -186,14 +196,17 @@ def bench_batch(args) -> dict:
                 h.update(j.msgs.tobytes())
             return h.hexdigest()
 
-        best, best_ms, table = autotune.tune(ctx, batched, key=f"batch_{n}", check=digest)
-        tuned = {"best": best, "best_ms": best_ms, "default": default_tune,
+        best, best_ms, table = autotune.tune(ctx, batched, key=autotune.BATCH_KEY, check=digest)
+        tuned = {"best": best, "best_ms": best_ms, "default": single_tune,
                  "default_ms": default_ms, "candidates": len(table)}
+    batch_tune = ctx.get_tune()
+    ctx.set_tune(**single_tune)
     t = time.perf_counter()
     reps_plain = 2
     for _ in range(reps_plain):
         plain()
     plain_s = (time.perf_counter() - t) / reps_plain
+    ctx.set_tune(**batch_tune)
     for _ in range(max(args.warmup, 1)):
         batched()
     t = time.perf_counter()
@@ -207,13 +220,15 @@ def bench_batch(args) -> dict:
                                   i.chal.itemsize) for i in insts)
     for d in dev:
         d.free()
+    ctx.close()
     return {
         "metric": "batched sumcheck table-elems/s (512 instances, n ~ U{12..22})",
         "value": elems / batch_s, "unit": "table-elems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": batch_s * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32 (BabyBear Montgomery / BabyBear^4)",
-        "data": "synthetic (seeded uniform field elements)",
+        "data": "synthetic (seeded uniform field elements; one distinct input and label "
+                "per instance, resident in HBM)",
         "config": {"workload": f"batch_{n}", "instances": n, "sizes": "U{12..22} seed 4",
                    "kinds": {"product": spec.kinds.count("product"),
                              "zerocheck": spec.kinds.count("zerocheck")},
@@ -223,47 +238,56 @@ def bench_batch(args) -> dict:
         "speedup_vs_plain": plain_s / batch_s,
         "proofs_per_s": n / batch_s, "plain_proofs_per_s": n / plain_s,
         "hbm_gbs": alg / batch_s / 1e9, "algorithmic_bytes": alg,
+        "roofline_frac": alg / batch_s / 1e9 / HBM_PEAK_GBS,
         "identical_to_plain": bool(identical), "verified": bool(verified),
-        "tune": ctx.get_tune(), "autotune": tuned,
+        "tune": batch_tune, "autotune": tuned,
     }
 
 
-def bench_serve(args) -> dict:
-    """Served proving: Poisson arrivals (2000/s, n ~ U{14..24}, degree-3
-    zerocheck over BabyBear^4) through the continuous-batching server, against
-    serving the same trace one proof per call, first come first served."""
+def bench_serve(args, pool=None) -> dict:
+    """Served proving: Poisson arrivals (2000/s over 60 s, n ~ U{14..24},
+    degree-3 zerocheck over BabyBear^4) through the continuous-batching
+    server, against serving the same trace one proof per call, first come
+    first served. Request i uses input i mod P of its size (a pool of P
+    distinct resident inputs per size) and a transcript label of its own;
+    latency runs from submission to completion (input upload excluded: the
+    requests reference device-resident tables)."""
     from zorch_amd import _lib
     from zorch_amd import workloads as W
-    from zorch_amd.runtime import DeviceTables, default_context
+    from zorch_amd.runtime import Context, DeviceTables
     from zorch_amd.serve import ProofServer
     from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
     from zorch_amd.sumcheck._device import make_job, prove_stage
 
     spec = W.serve_spec(rate=args.rate, duration=args.duration)
-    ctx = default_context()
+    per = args.serve_inputs
+    if pool is None:
+        pool = W.serve_inputs(spec, per_size=per)
+    ctx = Context(0)
     lib = _lib.lib()
-    # One device-resident input per size; every request has its own transcript.
-    base = {n: W.zerocheck_bb4(n, (2000 + n, 2100 + n), 2200 + n, 2300 + n)
-            for n in sorted(set(spec.sizes))}
-    dev = {n: DeviceTables.from_host(base[n].tables, ctx) for n in base}
+    dev = {n: [DeviceTables.from_host(i.tables, ctx) for i in insts] for n, insts in pool.items()}
     eq = _lib.ZS_MODE_EQ
+    budget = int(args.hbm_budget_gb * 1e9)
+
+    def inst_of(i):
+        n = spec.sizes[i]
+        return pool[n][i % per], dev[n][i % per]
 
     def transcript(i):
-        inst = base[spec.sizes[i]]
-        return Sha256FieldTranscript.new(W.label(1_000_000 + i), inst.transcript_field)
+        return Sha256FieldTranscript.new(W.serve_label(i), inst_of(i)[0].transcript_field)
 
     def submit(srv, i):
-        inst = base[spec.sizes[i]]
-        return srv.submit(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, transcript(i),
-                          inst.z)
+        inst, d = inst_of(i)
+        return srv.submit(d, inst.claim, inst.chal, eq, inst.degree, transcript(i), inst.z)
 
     def pct(xs, q):
         return float(np.percentile(np.asarray(xs) * 1e3, q)) if xs else None
 
     # Launch shapes of the served steps: the autotuner times a burst of the
     # trace's first requests through a server under each candidate (results
-    # never depend on them); the one-proof-per-call runs below use the
-    # context's defaults, tuned for single proofs.
+    # never depend on them) and caches the winner as "serve"; the
+    # one-proof-per-call runs use the context's defaults, tuned for single
+    # proofs.
     single_tune = ctx.get_tune()
     tuned = None
     if not args.no_autotune:
@@ -271,14 +295,14 @@ def bench_serve(args) -> dict:
         burst = min(400, len(spec.arrivals))
 
         def served_burst():
-            with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+            with ProofServer(ctx, budget, tune_key=None) as srv:
                 for rid in [submit(srv, j) for j in range(burst)]:
                     srv.wait(rid)
 
         cands = [dict(max_grid=g, pairs_per_thread=p_, tail_pairs=t, step_rounds=s)
                  for g in (256, 512, 1024) for p_ in (1, 4) for t in (1024, 2048, 4096)
                  for s in (0, 4)]
-        best, best_ms, _ = autotune.tune(ctx, served_burst, key="serve_zerocheck",
+        best, best_ms, _ = autotune.tune(ctx, served_burst, key=autotune.SERVE_KEY,
                                          candidates=cands, reps=2)
         tuned = {"best": best, "best_ms": best_ms, "burst": burst, "candidates": len(cands)}
     served_tune = ctx.get_tune()
@@ -286,7 +310,7 @@ def bench_serve(args) -> dict:
     # Open loop through the server over the whole trace.
     N = len(spec.arrivals)
     lat, sample = [], {}
-    with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+    with ProofServer(ctx, budget, tune_key=None) as srv:
         ids = {}
         i = done = 0
         t0 = time.perf_counter()
@@ -301,7 +325,7 @@ def bench_serve(args) -> dict:
                 res, l = srv.poll(rid)
                 lat.append(l)
                 k = ids.pop(rid)
-                if k % max(N // 16, 1) == 0:
+                if k % max(N // 16, 1) == 0 or spec.sizes[k] == 24 and len(sample) < 24:
                     sample[k] = res.msgs.words.copy()
                 done += 1
                 busy = True
@@ -309,27 +333,28 @@ def bench_serve(args) -> dict:
                 time.sleep(0.0002)
         served_s = time.perf_counter() - t0
         sst = srv.stats()
-    # The same requests alone give the same proofs.
+    # Sampled requests alone give the same proofs, and the verifier accepts them.
     identical = True
     for k, msgs in sample.items():
-        inst = base[spec.sizes[k]]
-        alone = prove_stage(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, transcript(k),
-                            inst.z, ctx=ctx)
+        inst, d = inst_of(k)
+        alone = prove_stage(d, inst.claim, inst.chal, eq, inst.degree, transcript(k), inst.z,
+                            ctx=ctx)
         identical &= bool((alone.msgs.words == msgs).all())
         identical &= _verify_proof(inst, msgs, transcript(k))
 
     # One proof per call, first come first served, over the first part of the trace.
     ctx.set_tune(**single_tune)
     Np = int(np.searchsorted(spec.arrivals, args.plain_duration))
-    jobs = {n: make_job(dev[n], base[n].claim, base[n].chal, eq, base[n].degree, base[n].z)
-            for n in base}
+    jobs = {(n, j): make_job(dev[n][j], pool[n][j].claim, pool[n][j].chal, eq,
+                             pool[n][j].degree, pool[n][j].z)
+            for n in pool for j in range(per)}
     plain_lat = []
     t0 = time.perf_counter()
     for j in range(Np):
         now = time.perf_counter() - t0
         if now < spec.arrivals[j]:
             time.sleep(spec.arrivals[j] - now)
-        job = jobs[spec.sizes[j]]
+        job = jobs[(spec.sizes[j], j % per)]
         st = transcript(j).state.copy()
         _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
                                 ctypes.byref(job.proof)))
@@ -343,10 +368,10 @@ def bench_serve(args) -> dict:
     ctx.set_tune(**served_tune)
     prepared = []
     for j in range(C):
-        inst = base[spec.sizes[j]]
-        prepared.append((make_job(dev[inst.n], inst.claim, inst.chal, eq, inst.degree, inst.z),
+        inst, d = inst_of(j)
+        prepared.append((make_job(d, inst.claim, inst.chal, eq, inst.degree, inst.z),
                          transcript(j)))
-    with ProofServer(ctx, int(args.hbm_budget_gb * 1e9)) as srv:
+    with ProofServer(ctx, budget, tune_key=None) as srv:
         t0 = time.perf_counter()
         cids = [srv.submit_job(job, tr) for job, tr in prepared]
         for rid in cids:
@@ -356,26 +381,29 @@ def bench_serve(args) -> dict:
     states = [tr.state.copy() for _, tr in prepared]
     t0 = time.perf_counter()
     for j in range(C):
-        job = jobs[spec.sizes[j]]
+        job = jobs[(spec.sizes[j], j % per)]
         _lib.check(lib.zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(states[j]),
                                 ctypes.byref(job.proof)))
     cap_plain = C / (time.perf_counter() - t0)
     elems = sum(2 * (1 << n) for n in spec.sizes)
-    for d in dev.values():
-        d.free()
+    for ds in dev.values():
+        for d in ds:
+            d.free()
+    ctx.close()
     return {
         "metric": "served proofs/s and p50/p99 latency (Poisson arrivals, n ~ U{14..24})",
         "value": N / served_s, "unit": "proofs/s", "n_gpus": 1,
         "steps": N, "warmup": 0, "ms_per_step": served_s / N * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32 (BabyBear Montgomery / BabyBear^4)",
-        "data": "synthetic (seeded uniform field elements; one input per size, one "
-                "transcript label per request)",
+        "data": f"synthetic (seeded uniform field elements; {per} distinct resident inputs "
+                "per size, one transcript label per request)",
         "config": {"workload": "serve_poisson", "rate_per_s": args.rate,
                    "duration_s": args.duration, "requests": N, "sizes": "U{14..24} seed 5",
                    "kind": "zerocheck eq(z,x) A B over babybearx4_mont",
                    "hbm_budget_gb": args.hbm_budget_gb, "parallelism": "single GPU"},
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": pct(lat, 100),
+        "latency_excludes": "input upload (requests reference device-resident tables)",
         "table_elems_per_s": elems / served_s,
         "server": {k: sst[k] for k in ("steps", "peak_active", "peak_resident_bytes",
                                        "busy_s", "failed")},
@@ -389,64 +417,65 @@ def bench_serve(args) -> dict:
     }
 
 
-def cpu_baseline(inst, threads: int) -> dict:
-    """The C oracle (the reference algorithm restated) on the host cores."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_run(inst, threads: int, budget_s: float) -> dict:
     from tests.oracle_glue import run_c
     t = time.perf_counter()
     reps = 0
     while True:
         run_c(inst, threads=threads)
         reps += 1
-        if time.perf_counter() - t > 10.0 or reps >= 64:
+        if time.perf_counter() - t > budget_s or reps >= 64:
             break
     el = time.perf_counter() - t
-    return {"value": inst.table_elems * reps / el, "unit": "table-elems/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{reps} full proof(s) of {inst.name} with the C oracle "
-                      f"({threads} OpenMP threads), {el:.1f} s"}
+    return {"value": inst.table_elems * reps / el, "proofs": reps, "seconds": round(el, 2),
+            "threads": threads}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="zerocheck_bb4_n24")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--batch-count", type=int, default=512)
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="batch workload: keep the default launch shapes")
-    ap.add_argument("--rate", type=float, default=2000.0)
-    ap.add_argument("--duration", type=float, default=60.0)
-    ap.add_argument("--plain-duration", type=float, default=10.0)
-    ap.add_argument("--capacity-requests", type=int, default=2000)
-    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="only run device copies of BYTES (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(each instance runs on one GPU).", file=sys.stderr)
-        return 2
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration, max(args.steps, 1))
-        return 0
-    if args.workload == "batch_512":
-        print(json.dumps(bench_batch(args)), flush=True)
-        return 0
-    if args.workload == "serve_poisson":
-        print(json.dumps(bench_serve(args)), flush=True)
-        return 0
+def cpu_baseline(inst) -> dict:
+    """The C oracle (the reference algorithm restated; the reference's own FRX
+    CPU path cannot run here) on the host cores: all of the box's share and
+    one thread, whole proofs of the same workload."""
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, 16)
+    allc = _cpu_run(inst, threads, 10.0)
+    one = _cpu_run(inst, 1, 10.0)
+    return {"value": allc["value"], "unit": "table-elems/s", "cores": threads, "kind": "port",
+            "sample": f"{allc['proofs']} full proof(s) of {inst.name} with the C oracle on "
+                      f"{threads} OpenMP threads ({allc['seconds']} s); "
+                      f"{one['proofs']} on one thread ({one['seconds']} s)",
+            "nproc": nproc, "cpu_model": _cpu_model(),
+            "all_core": allc, "one_thread": one}
 
+
+def hbm_peak(ctx) -> dict:
+    """Measured streaming bandwidth of this device (zs_hbm_bandwidth, 4 GiB)."""
     from zorch_amd import _lib
-    from zorch_amd.fields import babybearx4_mont
+    rd, cp = ctypes.c_double(), ctypes.c_double()
+    _lib.check(_lib.lib().zs_hbm_bandwidth(ctx.handle, 4 << 30, 5, ctypes.byref(rd),
+                                           ctypes.byref(cp)))
+    return {"read_gbs": rd.value, "copy_gbs": cp.value, "nominal_gbs": HBM_PEAK_GBS,
+            "bytes": 4 << 30, "method": "best of 5 streaming-read / streaming-copy kernels, "
+                                        "HIP events"}
+
+
+def bench_single(args, inst, peak: dict | None) -> dict:
+    """The single-proof part: one whole proof per step, tables resident."""
+    from zorch_amd import _lib
     from zorch_amd.runtime import DeviceTables, default_context, profile, profile_read
     from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
     from zorch_amd.sumcheck._device import make_job
     from zorch_amd.workloads import algorithmic_bytes
 
-    inst = build_instance(args.workload)
     field = inst.tables.field
     ctx = default_context()
     tables = DeviceTables.from_host(inst.tables, ctx)
@@ -467,26 +496,11 @@ def main() -> int:
         step()
     t = time.perf_counter()
     for _ in range(args.steps):
-        st = step()
+        step()
     elapsed = time.perf_counter() - t
     ms_per_step = elapsed / args.steps * 1e3
     value = inst.table_elems * args.steps / elapsed
-
-    # Verify the last proof on the host.
-    from zorch_amd.challenge import ChallengePolicy
-    from zorch_amd.fields import FieldArray
-    from zorch_amd.sumcheck.eq.stage import EqPolyVerifier, EqSumClaim
-    from zorch_amd.sumcheck.prover import ProductSummand
-    from zorch_amd.sumcheck.stage import SumClaim, SumcheckVerifier
-    from zorch_amd.sumcheck.verifier import SumcheckRound
-    proof = FieldArray(job.msgs.copy(), inst.chal)
-    pol = ChallengePolicy(inst.chal)
-    if mode == _lib.ZS_MODE_EQ:
-        ok = EqPolyVerifier(ProductSummand(inst.degree), challenges=pol).verify(
-            EqSumClaim(inst.z, inst.claim, inst.n), proof, t0).ok
-    else:
-        ok = SumcheckVerifier(SumcheckRound(inst.degree, pol)).verify(
-            SumClaim(inst.claim, inst.n), proof, t0).ok
+    ok = _verify_proof(inst, job.msgs.copy(), t0)
 
     # Kernel timing with HIP events on the prover's stream (a separate pass).
     profile(ctx, True)
@@ -503,15 +517,17 @@ def main() -> int:
     dom = max(groups, key=lambda g: g["bytes_per_launch"] * g["launches_per_proof"])
     achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    tfile = ROOT / "profiles" / f"traffic_{inst.name}.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("kernels", {}).get(dom["symbol"])
     wt = inst.tables.field.itemsize
     wc = inst.chal.itemsize
     alg = algorithmic_bytes(inst.tables.shape[0], inst.n, wt, wc)
     kernel_bytes = int(sum(g["bytes_per_launch"] * g["launches_per_proof"] for g in groups))
-
-    out = {
+    proof_gbs = alg / (ms_per_step * 1e-3) / 1e9
+    slowest = max(groups, key=lambda g: g["total_ms"])
+    tables.free()
+    return {
         "metric": "sumcheck table-elems/s + HBM GB/s at 2^24; batched/served p50/p99 proof latency",
         "value": value,
         "unit": "table-elems/s",
@@ -529,7 +545,7 @@ def main() -> int:
                    "transcript": f"sha256 over {inst.transcript_field.name}",
                    "message_degree": inst.degree + (1 if mode == _lib.ZS_MODE_EQ else 0),
                    "parallelism": "single GPU"},
-        "proof_hbm_gbs": alg / (ms_per_step * 1e-3) / 1e9,
+        "proof_hbm_gbs": proof_gbs,
         "proof_algorithmic_bytes": alg,
         # What the launched kernels move per proof (the model above counts a
         # fold-and-evaluate pass per round; rounds 0-2 here read the input
@@ -541,13 +557,87 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom["symbol"], "launches_per_proof": dom["launches_per_proof"],
                      "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
-                     "avg_ms": dom["avg_ms"]},
+                     "avg_ms": dom["avg_ms"],
+                     # The whole proof against the same peak (the figure the
+                     # 70 % target is stated on) and against the measured one.
+                     "proof": {"achieved": proof_gbs, "frac": proof_gbs / HBM_PEAK_GBS,
+                               "frac_of_measured_read_peak":
+                                   proof_gbs / peak["read_gbs"] if peak else None,
+                               "ms": ms_per_step},
+                     "slowest_kernel": {"symbol": slowest["symbol"],
+                                        "ms_per_proof": slowest["total_ms"]}},
         "kernels": {"per_round": per_round, "by_symbol": groups,
                     "finish_avg_ms": float(np.mean(fin)) if fin else None},
     }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="zerocheck_bb4_n24")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="leave out the batched part")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the served part")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched / served parts: keep the default launch shapes")
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--plain-duration", type=float, default=10.0)
+    ap.add_argument("--capacity-requests", type=int, default=2000)
+    ap.add_argument("--serve-inputs", type=int, default=4,
+                    help="distinct resident inputs per size in the served trace")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of BYTES (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(each instance runs on one GPU).", file=sys.stderr)
+        return 2
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(args.steps, 1))
+        return 0
+    if args.workload == "batch_512":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return 0
+    if args.workload == "serve_poisson":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return 0
+
+    from zorch_amd import workloads as W
+    from zorch_amd.runtime import default_context
+
+    # Inputs of every part, generated on host threads before any timing.
+    t = time.perf_counter()
+    inst = build_instance(args.workload)
+    full = args.workload == "zerocheck_bb4_n24"
+    batch_insts = None
+    if full and not args.no_batch:
+        batch_insts = W.batch_instances(W.batch_spec(count=args.batch_count))
+    pool = None
+    if full and not args.no_serve:
+        pool = W.serve_inputs(W.serve_spec(rate=args.rate, duration=args.duration),
+                              per_size=args.serve_inputs)
+    print(f"inputs generated in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+
+    peak = hbm_peak(default_context())
+    out = bench_single(args, inst, peak)
+    out["hbm_peak"] = peak
+    print("single proof done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(inst, threads)
+        out["cpu_baseline"] = cpu_baseline(inst)
+        print("cpu baseline done", file=sys.stderr, flush=True)
+    if batch_insts is not None:
+        out["batch"] = bench_batch(args, batch_insts)
+        del batch_insts
+        print("batched part done", file=sys.stderr, flush=True)
+    if pool is not None:
+        out["serve"] = bench_serve(args, pool)
+        print("served part done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

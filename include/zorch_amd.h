@@ -167,6 +167,16 @@ int zs_memcpy_dtoh(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
  * copy of known size that HBM byte counters are calibrated against. */
 int zs_memcpy_dtod(zs_ctx* ctx, void* dst, const void* src, size_t bytes);
 
+/* ---- HBM bandwidth (measurement aid) ---------------------------------------
+ * No counterpart in the reference. The attainable HBM bandwidth of this
+ * device, the peak the streaming rounds are judged against: a streaming-read
+ * kernel (16-byte loads, every byte read once) over `bytes` of device memory
+ * from the context's pool, and a streaming copy of the same size (read and
+ * write, counted twice), each the best of `reps` runs timed with HIP events
+ * on the context's stream; GB/s = 1e9 bytes per second. */
+int zs_hbm_bandwidth(zs_ctx* ctx, size_t bytes, uint32_t reps, double* read_gbs,
+                     double* copy_gbs);
+
 /* ---- kernel timing -------------------------------------------------------
  * Profiling aid with no counterpart in the reference (whose only tracing is
  * XLA's dump flags): while enabled, every pass, finisher and tail kernel the

@@ -121,3 +121,43 @@ def test_batch_of_one_and_empty_batch():
     (res,) = _batch([inst])
     _same(res, _single(inst))
     assert prove_stage_batch([], []) == []
+
+
+def test_batch_512_spec_at_full_size():
+    """The batched configuration as stated: 512 instances, n ~ U{12..22} from
+    seed 4, product or zerocheck with probability 1/2, BabyBear^4 challenges,
+    device-resident tables, one zs_prove_batch. Every member equals its
+    one-call proof and the verifier accepts it; a seeded sample, the n >= 21
+    members among it, equals the C oracle."""
+    import numpy as np
+
+    from zorch_amd.challenge import ChallengePolicy
+    from zorch_amd.sumcheck.eq.stage import EqPolyVerifier, EqSumClaim
+    from zorch_amd.sumcheck.prover import ProductSummand
+    from zorch_amd.sumcheck.stage import SumClaim, SumcheckVerifier
+    from zorch_amd.sumcheck.verifier import SumcheckRound
+
+    spec = W.batch_spec(count=512)
+    insts = W.batch_instances(spec)
+    ctx = default_context()
+    dev = [DeviceTables.from_host(i.tables, ctx) for i in insts]
+    got = _batch(insts, dev)
+    for res, inst, d in zip(got, insts, dev):
+        single = prove_stage(d, inst.claim, inst.chal, _mode(inst), inst.degree, _t0(inst), inst.z)
+        _same(res, single)
+        pol = ChallengePolicy(inst.chal)
+        if inst.mode == "zerocheck":
+            ok = EqPolyVerifier(ProductSummand(inst.degree), challenges=pol).verify(
+                EqSumClaim(inst.z, inst.claim, inst.n), res.msgs, _t0(inst)).ok
+        else:
+            ok = SumcheckVerifier(SumcheckRound(inst.degree, pol)).verify(
+                SumClaim(inst.claim, inst.n), res.msgs, _t0(inst)).ok
+        assert ok, inst.name
+    rng = np.random.default_rng(4)
+    big = [i for i, inst in enumerate(insts) if inst.n >= 21]
+    sample = set(rng.choice(big, size=min(6, len(big)), replace=False).tolist())
+    sample |= set(rng.choice(len(insts), size=6, replace=False).tolist())
+    for i in sorted(sample):
+        _matches_oracle(got[i], insts[i])
+    for d in dev:
+        d.free()

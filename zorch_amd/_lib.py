@@ -171,6 +171,8 @@ SIGNATURES = {
     "zs_prove_check": (_INT, [ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                               ctypes.POINTER(_U32)]),
     "zs_field_op": (_INT, [_P, _U32, _U32, _P, _P, _P, _SZ]),
+    "zs_hbm_bandwidth": (_INT, [_P, _SZ, _U32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
     "zs_prove": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                         ctypes.POINTER(ProofC)]),
     "zs_prove_noalloc": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),

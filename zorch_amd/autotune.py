@@ -68,6 +68,27 @@ def apply(ctx: Context, key: str) -> dict | None:
     return params
 
 
+# Cache keys of the workload classes the library applies by itself: batched
+# proving (``prove_stage_batch``) and served proving (``ProofServer``).
+BATCH_KEY = "batch"
+SERVE_KEY = "serve"
+
+
+class applied:
+    """Context manager: the cached shape of `key` on `ctx` for the duration
+    (the context's own shape when none is cached), restored afterwards."""
+
+    def __init__(self, ctx: Context, key: str):
+        self.ctx, self.key, self.saved = ctx, key, None
+
+    def __enter__(self) -> dict | None:
+        self.saved = self.ctx.get_tune()
+        return apply(self.ctx, self.key)
+
+    def __exit__(self, *exc) -> None:
+        self.ctx.set_tune(**self.saved)
+
+
 def tune(ctx: Context, run: Callable[[], None], key: str | None = None,
          candidates: Iterable[dict] = CANDIDATES, reps: int = 3,
          check: Callable[[], object] | None = None) -> tuple[dict, float, list]:

@@ -289,6 +289,33 @@ static int field_op_typed(zs_ctx* ctx, uint32_t op, const void* a, const void* b
   return e == hipSuccess ? ZS_OK : hip_fail(e, "zs_field_op");
 }
 
+// Streaming read of n 16-byte words: four independent loads in flight per
+// thread and step; the xor of everything read is stored only when it equals
+// a constant, which keeps the loads without writing a byte in practice.
+__global__ __launch_bounds__(256) void stream_read_kernel(const uint4* __restrict__ p, size_t n,
+                                                          uint32_t* sink) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t acc = 0;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = p[i], b = p[i + stride], c = p[i + 2 * stride], d = p[i + 3 * stride];
+    acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^
+           d.z ^ d.w;
+  }
+  for (; i < n; i += stride) {
+    const uint4 a = p[i];
+    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  if (acc == 0x9e3779b9u) sink[blockIdx.x] = acc;
+}
+
+// Streaming copy of n 16-byte words.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src,
+                                                          uint4* __restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace zs
 
 using namespace zs;
@@ -706,6 +733,60 @@ int zs_field_op(zs_ctx* ctx, uint32_t field, uint32_t op, const void* a, const v
     case ZS_F_GOLDILOCKS: return field_op_typed<GL>(ctx, op, a, b, out, count);
   }
   return fail(ZS_EINVAL, "unknown field id");
+}
+
+int zs_hbm_bandwidth(zs_ctx* ctx, size_t bytes, uint32_t reps, double* read_gbs,
+                     double* copy_gbs) {
+  if (!ctx || !read_gbs || !copy_gbs || reps == 0) return fail(ZS_EINVAL, "bad argument");
+  bytes &= ~(size_t)4095;
+  if (bytes < ((size_t)1 << 20)) return fail(ZS_EINVAL, "measure at least 1 MiB");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  int dev = 0, cus = 0;
+  ZS_HIP(hipGetDevice(&dev));
+  ZS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t grid = (uint32_t)cus * 8;
+  void *a = nullptr, *b = nullptr, *sink = nullptr;
+  if ((rc = ctx->pool->alloc(bytes, &a))) return rc;
+  if ((rc = ctx->pool->alloc(bytes, &b))) {
+    ctx->pool->release(a);
+    return rc;
+  }
+  if ((rc = ctx->pool->alloc((size_t)grid * 4, &sink))) {
+    ctx->pool->release(a);
+    ctx->pool->release(b);
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0 = ctx->event(), e1 = ctx->event();
+  const size_t n = bytes / 16;
+  double best_read = 0, best_copy = 0;
+  hipError_t e = hipMemsetAsync(a, 0x5a, bytes, s);
+  for (uint32_t r = 0; r < reps + 1 && e == hipSuccess; ++r) {  // the first run warms up
+    (void)hipEventRecord(e0, s);
+    stream_read_kernel<<<grid, 256, 0, s>>>((const uint4*)a, n, (uint32_t*)sink);
+    (void)hipEventRecord(e1, s);
+    e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && r > 0 && ms > 0) best_read = std::max(best_read, bytes / (ms * 1e6));
+    (void)hipEventRecord(e0, s);
+    stream_copy_kernel<<<grid, 256, 0, s>>>((const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && r > 0 && ms > 0)
+      best_copy = std::max(best_copy, 2.0 * bytes / (ms * 1e6));
+  }
+  ctx->spare_events.push_back(e0);
+  ctx->spare_events.push_back(e1);
+  ctx->pool->release(a);
+  ctx->pool->release(b);
+  ctx->pool->release(sink);
+  if (e != hipSuccess) return hip_fail(e, "zs_hbm_bandwidth");
+  *read_gbs = best_read;
+  *copy_gbs = best_copy;
+  return ZS_OK;
 }
 
 int zs_verify_round(uint32_t chal_field, zs_transcript* t, const void* claim, const void* msg,

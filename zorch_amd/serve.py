@@ -27,9 +27,16 @@ class ProofServer:
     context's stream and launch shapes until ``close``. Inputs resident on
     another context of the same device may be submitted."""
 
-    def __init__(self, ctx: Context | None = None, hbm_budget_bytes: int = 0):
+    def __init__(self, ctx: Context | None = None, hbm_budget_bytes: int = 0,
+                 tune_key: str | None = "serve"):
+        from zorch_amd import autotune
+
         self._own_ctx = ctx is None
         self.ctx = ctx if ctx is not None else Context(0, hbm_budget_bytes)
+        # The launch shapes the autotuner cached for served proving, if any
+        # (the server owns the context's shapes; results never depend on them).
+        if tune_key is not None:
+            autotune.apply(self.ctx, tune_key)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().zs_server_create(self.ctx.handle, hbm_budget_bytes,
                                                ctypes.byref(h)))

@@ -99,11 +99,21 @@ def prove_stage(tables, claim: FieldArray, chal_field: Any, mode: int, degree: i
 
 
 def prove_stage_batch(jobs: Sequence[StageJob], transcripts: Sequence[Any],
-                      ctx: Context | None = None) -> list[DeviceProof]:
+                      ctx: Context | None = None,
+                      tune_key: str | None = "batch") -> list[DeviceProof]:
+    """Every job in one ``zs_prove_batch``; the launch shapes cached by the
+    autotuner under `tune_key` apply for the call (results never depend on
+    them)."""
+    from zorch_amd import autotune
+
     ctx = ctx or default_context()
     n = len(jobs)
     args = (_lib.ProveArgsC * n)(*[j.args for j in jobs])
     sts = (_lib.TranscriptC * n)(*[t.state.copy() for t in transcripts])
     proofs = (_lib.ProofC * n)(*[j.proof for j in jobs])
-    _lib.check(_lib.lib().zs_prove_batch(ctx.handle, n, args, sts, proofs))
+    if tune_key is None:
+        _lib.check(_lib.lib().zs_prove_batch(ctx.handle, n, args, sts, proofs))
+    else:
+        with autotune.applied(ctx, tune_key):
+            _lib.check(_lib.lib().zs_prove_batch(ctx.handle, n, args, sts, proofs))
     return [_result(j, sts[i].copy(), transcripts[i]) for i, j in enumerate(jobs)]

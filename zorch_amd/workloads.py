@@ -193,6 +193,20 @@ def batch_instance(spec: BatchSpec, i: int) -> Instance:
     return zerocheck_bb4(n, (base, base + 1), base + 2, base + 3)
 
 
+def _workers(workers: int | None) -> int:
+    import os
+    return workers or max(1, min(16, os.cpu_count() or 1))
+
+
+def batch_instances(spec: BatchSpec, workers: int | None = None) -> list[Instance]:
+    """Every instance of a batch spec, generated on host threads (numpy
+    releases the interpreter lock in its array kernels)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from functools import partial
+    with ThreadPoolExecutor(_workers(workers)) as ex:
+        return list(ex.map(partial(batch_instance, spec), range(len(spec.sizes))))
+
+
 @dataclass
 class ServeSpec:
     arrivals: np.ndarray        # seconds
@@ -211,6 +225,32 @@ def serve_spec(rate: float = 2000.0, duration: float = 60.0, seed: int = 5, lo: 
     t = t[t < duration]
     sizes = [int(x) for x in rng.integers(lo, hi + 1, size=t.size)]
     return ServeSpec(t, sizes, rate, duration, seed)
+
+
+def serve_inputs(spec: ServeSpec, per_size: int = 4,
+                 workers: int | None = None) -> dict[int, list[Instance]]:
+    """A pool of `per_size` distinct zerocheck inputs for every size in a serve
+    spec (request i uses input i mod per_size of its size, with a transcript
+    label of its own, ``serve_label(i)``)."""
+    from concurrent.futures import ThreadPoolExecutor
+    keys = [(n, j) for n in sorted(set(spec.sizes)) for j in range(per_size)]
+
+    def make(key):
+        n, j = key
+        base = 20_000 + 100 * n + 4 * j
+        return zerocheck_bb4(n, (base, base + 1), base + 2, base + 3)
+
+    with ThreadPoolExecutor(_workers(workers)) as ex:
+        made = list(ex.map(make, keys))
+    pool: dict[int, list[Instance]] = {}
+    for (n, _), inst in zip(keys, made):
+        pool.setdefault(n, []).append(inst)
+    return pool
+
+
+def serve_label(i: int) -> bytes:
+    """Transcript label of request i of a serve trace."""
+    return label(1_000_000 + i)
 
 
 # Algorithmic HBM bytes of one proof under the fused-pass model: pass 0 reads
