@@ -23,6 +23,7 @@ from oracle import fields as OF
 from oracle import sha256 as OSHA
 from oracle import sumcheck as OS
 from oracle import transcript as OT
+from tests import blake3_indep as B3I
 from tests.golden.recipes import RECIPES, build, fixture_state
 from tests.oracle_glue import run_c, run_py
 
@@ -79,6 +80,36 @@ def test_blake3_stream_equals_tree(length):
         pos += k
     assert s.finalize(150) == want
     assert OB3.blake3(data, 32) == want[:32]  # the hash is a prefix of the XOF
+
+
+def test_independent_blake3_pinned_to_published_vectors():
+    assert B3I.blake3_xof(b"") == _B3_EMPTY
+    assert B3I.blake3_xof(bytes(range(3)), 64) == _B3_LEN3_XOF64
+
+
+@pytest.mark.parametrize("length", [0, 1, 3, 63, 64, 65, 128, 1023, 1024, 1025, 2048, 2049,
+                                    3072, 3073, 4095, 5120, 6145, 7169, 8192, 8193,
+                                    11264, 16385, 31744, 33793])
+def test_blake3_oracle_equals_independent_restatement(length):
+    # Multi-block chunks, parent nodes, trees of 1..33 chunks (odd carries at
+    # several levels) and extendable output over four output blocks, against
+    # a restatement that shares no code with the oracle.
+    data = bytes((i * 7 + 3) % 251 for i in range(length))
+    assert OB3.blake3(data, 250) == B3I.blake3_xof(data, 250)
+
+
+def test_independent_restatement_catches_a_planted_parent_slip(monkeypatch):
+    # A parent node compressed with the chunk counter instead of 0 (a plausible
+    # slip) changes every input of more than one chunk, so the comparison
+    # above would see it.
+    data = bytes(i % 251 for i in range(2049))
+    good = OB3.blake3(data, 32)
+    real = OB3._parent
+    monkeypatch.setattr(OB3, "_parent",
+                        lambda l, r: OB3.Output(OB3.IV, real(l, r).block, 1, OB3.BLOCK_LEN,
+                                                OB3.PARENT))
+    assert OB3.blake3(data, 32) != good
+    assert OB3.blake3(data[:1024], 32) == B3I.blake3_xof(data[:1024], 32)
 
 
 def test_blake3_transcript_framing():
