@@ -124,7 +124,13 @@ typedef struct zs_prove_args {
   uint32_t rounds;           /* n: every table has 2^n entries */
   const void* tables;        /* k x 2^n elements, table-major */
   uint32_t tables_on_device; /* 1: `tables` is a device pointer */
-  uint32_t reserved;
+  uint32_t refold_rounds;    /* ZS_MODE_PRODUCT, zs_prove and zs_state_* only: the first
+                                refold_rounds rounds (at most rounds / 2) never fold the
+                                input; each refolds it over the bound prefix with the
+                                running eq(r[<i], .) table (the sqrt-space engine,
+                                zorch/sumcheck/sqrt_space.py:40-132, which binds
+                                rounds / 2 that way). The same messages as folding;
+                                0: fold every round */
   const void* eq_point;      /* ZS_MODE_EQ: n challenge-field elements (host) */
   const void* claim;         /* one challenge-field element (host) */
   const zs_wire* wire;       /* message wire form; null: ZS_WIRE_VALUES (copied) */
@@ -279,6 +285,52 @@ int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
 #define ZS_NOALLOC_SCRATCH_BYTES (64u << 20)
 int zs_prove_noalloc(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript,
                      zs_proof* out);
+
+/* ---- round-cost engines ---------------------------------------------------
+ * The sqrt-space engine is zs_prove with refold_rounds = rounds / 2
+ * (prove_sqrt_space, zorch/sumcheck/sqrt_space.py:99-132).
+ *
+ * Univariate skip: prove_univariate_skip (zorch/sumcheck/univariate_skip.py:
+ * 136-190) for skip_rounds in [1, rounds] (skip_rounds 0 is zs_prove). The
+ * first skip_rounds variables index the order-2^skip_rounds two-adic subgroup;
+ * round 0 sends msg0, the ascending coefficients of
+ * s0(Z) = sum_x prod_k f_k(Z, x) (degree * (2^skip_rounds - 1) + 1 table-field
+ * elements, written to msg0; round0_message, :65-101), observed in the table
+ * field, and samples r0; the prism is bound at r0 and the remaining
+ * rounds - skip_rounds rounds are product rounds over challenge-field tables
+ * (messages on args->wire, natural values when null). out->msgs holds those
+ * rounds' messages, out->point r0 then their challenges (1 + rounds -
+ * skip_rounds raw words), out->value and out->final_tables as zs_prove.
+ * ZS_MODE_PRODUCT; tables BabyBear or Goldilocks, the transcript over the
+ * tables' field. The subgroup's generator is the field's multiplicative
+ * generator (BabyBear 31, Goldilocks 7) to the power (p - 1) / 2^k. */
+int zs_prove_univariate_skip(zs_ctx* ctx, const zs_prove_args* args, uint32_t skip_rounds,
+                             zs_transcript* transcript, void* msg0, zs_proof* out);
+/* Round 0 of the skip alone (skip_round0, zorch/sumcheck/univariate_skip.py:
+ * 104-133): msg0, the raw challenge r0 (chal), s0(r0) (reduced) and the prism
+ * bound at r0 in `bound_tables`, a device buffer of num_tables x
+ * 2^(rounds - skip_rounds) challenge-field elements (table-major), from which
+ * any product tail can continue (zs_prove / zs_state_* with tables_on_device). */
+int zs_univariate_skip_round0(zs_ctx* ctx, const zs_prove_args* args, uint32_t skip_rounds,
+                              zs_transcript* transcript, void* msg0, void* chal, void* reduced,
+                              void* bound_tables);
+
+/* Small-value eq-poly sumcheck: prove_eq_poly_small_value
+ * (zorch/sumcheck/eq/small_value.py:148-192) with l0 in [1, rounds - rounds/2]:
+ * one pass over the tables precomputes the accumulators of rounds 1..l0
+ * (zorch/sumcheck/eq/accumulators.py:41-91), whose rounds are then
+ * contractions; the factors are refolded once over the bound prefix and the
+ * remaining rounds (the transition round first) are eq-poly rounds from the
+ * bound eq mass. Every message is on the compressed U-hat_degree domain (the
+ * reference's only wire for this engine: args->wire must be null); the
+ * messages equal those of EqPolyRound's (a v or y index of the accumulators is
+ * read most-significant digit first: the reference writes it
+ * least-significant first, which agrees up to l0 = 2). ZS_MODE_EQ,
+ * degree 2..4, one transcript word a challenge (the transcript over the
+ * challenge field, as SmallValueRound's observe_and_sample(msg, 1)). Outputs
+ * as zs_prove. */
+int zs_prove_small_value(zs_ctx* ctx, const zs_prove_args* args, uint32_t l0,
+                         zs_transcript* transcript, zs_proof* out);
 
 /* Many independent instances (uneven sizes, mixed kinds) in one submission;
  * instance i uses args[i], transcripts[i], outs[i], with the same results as

@@ -130,6 +130,32 @@ def run_py(inst) -> tuple[OracleResult, OS.OracleProof]:
     return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot, state), pf
 
 
+def oracle_inputs(inst):
+    """(table field, challenge field, transcript field, tables, claim, fresh
+    transcript) of an instance in the oracle's representation."""
+    tf, cf, trf = (OFIELD[inst.tables.field.name], OFIELD[inst.chal.name],
+                   OFIELD[inst.transcript_field.name])
+    tabs = [[tf.from_words([int(w) for w in inst.tables.words[q, x]])
+             for x in range(inst.tables.shape[1])] for q in range(inst.tables.shape[0])]
+    claim = cf.from_words([int(w) for w in inst.claim.words.reshape(-1)])
+    tcls = OT.OracleBlake3Transcript if inst.transcript_kind == "blake3" else OT.OracleTranscript
+    return tf, cf, trf, tabs, claim, tcls.new(inst.label, trf.item_bytes)
+
+
+def proof_result(inst, cf, pf: OS.OracleProof, eq: bool = False) -> OracleResult:
+    """An oracle proof as the storage words the device returns."""
+    wd = np.uint64 if cf is OF.GOLDILOCKS else np.uint32
+    msgs = np.array([[cf.to_words(m) for m in row] for row in pf.msgs], dtype=wd)
+    point = np.array(pf.point_words, dtype=wd)
+    value = np.array(cf.to_words(pf.value), dtype=wd)
+    fin = [cf.to_words(t[0]) for t in pf.final_tables]
+    if eq:
+        fin.append(cf.to_words(pf.eq_prev))
+    state = pf.transcript.state()
+    h, pend, tot = state if inst.transcript_kind == "sha256" else ((), b"", 0)
+    return OracleResult(msgs, point, value, np.array(fin, dtype=wd), h, pend, tot, state)
+
+
 def oracle_wire(inst, wire_py) -> tuple:
     """(F, wire) of the oracle for a package wire description: ("values",),
     ("domain", EvalDomain), ("coeffs",) or ("compressed",)."""

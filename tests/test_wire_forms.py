@@ -225,7 +225,7 @@ def _args(k=2, n=4, mode=_lib.ZS_MODE_PRODUCT, wire=None, rounds=None, tf=_lib.Z
           cf=_lib.ZS_F_BABYBEAR):
     claim = np.zeros(4, np.uint32)
     return _lib.ProveArgsC(table_field=tf, chal_field=cf, mode=mode, degree=k, num_tables=k,
-                           rounds=rounds or n, tables=1, tables_on_device=1, reserved=0,
+                           rounds=rounds or n, tables=1, tables_on_device=1, refold_rounds=0,
                            eq_point=1 if mode == _lib.ZS_MODE_EQ else None,
                            claim=claim.ctypes.data, wire=wire, eq_mass=None), claim
 

@@ -85,7 +85,7 @@ class ProveArgsC(ctypes.Structure):
         ("rounds", ctypes.c_uint32),
         ("tables", ctypes.c_void_p),
         ("tables_on_device", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("refold_rounds", ctypes.c_uint32),
         ("eq_point", ctypes.c_void_p),
         ("claim", ctypes.c_void_p),
         ("wire", ctypes.POINTER(WireC)),
@@ -179,6 +179,12 @@ SIGNATURES = {
                                 ctypes.POINTER(ProofC)]),
     "zs_prove_batch": (_INT, [_P, _SZ, ctypes.POINTER(ProveArgsC), ctypes.POINTER(TranscriptC),
                               ctypes.POINTER(ProofC)]),
+    "zs_prove_univariate_skip": (_INT, [_P, ctypes.POINTER(ProveArgsC), _U32,
+                                        ctypes.POINTER(TranscriptC), _P, ctypes.POINTER(ProofC)]),
+    "zs_univariate_skip_round0": (_INT, [_P, ctypes.POINTER(ProveArgsC), _U32,
+                                         ctypes.POINTER(TranscriptC), _P, _P, _P, _P]),
+    "zs_prove_small_value": (_INT, [_P, ctypes.POINTER(ProveArgsC), _U32,
+                                    ctypes.POINTER(TranscriptC), ctypes.POINTER(ProofC)]),
     "zs_state_create": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(_P)]),
     "zs_state_round": (_INT, [_P, ctypes.POINTER(TranscriptC), _P, _P, _P]),
     "zs_state_bound": (_INT, [_P, ctypes.POINTER(_U32)]),

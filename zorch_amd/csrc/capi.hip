@@ -129,8 +129,8 @@ static bool known_kind(uint32_t kind) { return kind == ZS_TR_SHA256 || kind == Z
 
 // One Fiat-Shamir hop (observe under slice framing, sample `limbs` words) on
 // either transcript kind; false if a BLAKE3 stream outgrows its subtree stack.
-static bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, size_t item,
-                           size_t limbs, uint8_t* out) {
+bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, size_t item, size_t limbs,
+                    uint8_t* out) {
   if (t->kind == ZS_TR_BLAKE3)
     return b3_fs_hop(*reinterpret_cast<Blake3State*>(&t->blake3), msg, len, item, limbs, out);
   sha256_fs_hop(*reinterpret_cast<Sha256State*>(&t->sha), msg, len, item, limbs, out);
@@ -138,7 +138,7 @@ static bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, siz
 }
 
 // Base prime field of a field id.
-static uint32_t base_of(uint32_t f) { return f == ZS_F_BABYBEAR4 ? ZS_F_BABYBEAR : f; }
+uint32_t base_of(uint32_t f) { return f == ZS_F_BABYBEAR4 ? ZS_F_BABYBEAR : f; }
 
 int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts,
                   uint32_t* mlen) {
@@ -158,6 +158,10 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
     return fail(ZS_EINVAL, "rounds must be in [1, " + std::to_string(ZS_MAX_ROUNDS) + "]");
   if (!a.tables || !a.claim) return fail(ZS_EINVAL, "null tables or claim");
   if (a.mode == ZS_MODE_EQ && !a.eq_point) return fail(ZS_EINVAL, "eq mode needs an equality point");
+  if (a.refold_rounds > a.rounds / 2)
+    return fail(ZS_EINVAL, "refold_rounds must be at most rounds / 2");
+  if (a.refold_rounds && a.mode != ZS_MODE_PRODUCT)
+    return fail(ZS_EINVAL, "refold rounds (the sqrt-space engine) take the product summand");
   if (t) {
     if (!known_kind(t->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
     const uint32_t tf = t->field;
@@ -212,7 +216,7 @@ Instance* make_instance(uint32_t tf, uint32_t cf, uint32_t k, uint32_t npts, boo
 }
 #undef ZS_DISPATCH
 
-static int set_device(zs_ctx* ctx) {
+int set_device(zs_ctx* ctx) {
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   return ZS_OK;
@@ -634,6 +638,8 @@ int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
   for (size_t i = 0; i < count; ++i) {
     uint32_t npts = 0;
     if ((rc = validate_args(args[i], &transcripts[i], &npts))) return rc;
+    if (args[i].refold_rounds)
+      return fail(ZS_EINVAL, "batched proving folds every round (refold_rounds must be 0)");
     const bool eq = args[i].mode == ZS_MODE_EQ;
     auto key = std::make_tuple(args[i].table_field, args[i].chal_field, args[i].num_tables, npts, eq);
     auto& g = groups[key];

@@ -16,6 +16,7 @@
 
 #include <vector>
 
+#include "engines.h"
 #include "kernels.h"
 #include "runtime.h"
 #include "wire.h"
@@ -134,6 +135,9 @@ class InstanceImpl final : public Instance {
     }
     const size_t work_stride = N / 2 > 0 ? N / 2 : 1;
     const size_t o_work = take((size_t)K * work_stride * sizeof(TC));
+    // The sqrt-space engine's two eq(r[<i], .) buffers (2^h entries each).
+    refold_ = a.refold_rounds;
+    const size_t o_sq = refold_ ? take(((size_t)2 << refold_) * sizeof(TC)) : 0;
     size_t o_in = 0;
     const bool stage_in = !a.tables_on_device;
     if (stage_in) o_in = take((size_t)K * N * sizeof(TT));
@@ -172,6 +176,8 @@ class InstanceImpl final : public Instance {
     args_.eq_l_fam = EQ ? (const TC*)(b + o_eql) : nullptr;
     args_.eq_r_fam = EQ ? (const TC*)(b + o_eqr) : nullptr;
     args_.trace = ctx->trace;
+    args_.sq_eq = refold_ ? (TC*)(b + o_sq) : nullptr;
+    args_.sq_half = refold_;
     if (stage_in) {
       ZS_HIP(hipMemcpyAsync((void*)args_.in, a.tables, (size_t)K * N * sizeof(TT),
                             hipMemcpyHostToDevice, s));
@@ -248,6 +254,30 @@ class InstanceImpl final : public Instance {
   }
 
   int enqueue(uint32_t first, uint32_t last, hipStream_t s) override {
+    if (refold_ && first < refold_) {
+      // Sqrt-space rounds: refold the untouched input over the bound prefix;
+      // the last one leaves its round table for the dense rounds after it.
+      const uint32_t stop = last < refold_ ? last : refold_;
+      const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
+      for (uint32_t r = first; r < stop; ++r) {
+        const size_t pairs = (size_t)1 << (n_ - r - 1);
+        const uint32_t ps = refold_pshift(pairs, maxg);
+        size_t grid = pairs >> ps;
+        if (grid > maxg) grid = maxg;
+        const bool write = r + 1 == refold_ && refold_ >= 2;
+        hipEvent_t ev = begin(s);
+        refold_pass_kernel<FT, FC, K><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(
+            args_, r, ps, write ? 1u : 0u, r + 1 == last ? 1u : 0u);
+        ZS_HIP(hipGetLastError());
+        if (ev) {
+          uint64_t bytes = (uint64_t)K * ((uint64_t)1 << n_) * sizeof(TT);
+          if (write) bytes += (uint64_t)K * 2 * pairs * sizeof(TC);
+          mark(s, r, KREC_REFOLD, (uint32_t)grid, bytes, ev);
+        }
+      }
+      if (stop == last) return ZS_OK;
+      first = stop;
+    }
     bool cont = false;  // the previous launch left this round's coefficients
     uint32_t r0 = first;
     if (fused_first_rounds(first, last)) {
@@ -466,10 +496,33 @@ class InstanceImpl final : public Instance {
       for (size_t i = 0; i < tmp.size(); ++i) o[i] = Lift<FT, FC>::go(tmp[i]);
       return ZS_OK;
     }
+    if (refold_ && bound < refold_)
+      return fail(ZS_EINVAL, "a sqrt-space state holds no round tables before round " +
+                                 std::to_string(refold_));
+    // The device holds the previous round's tables (the input after round
+    // 0): fold a copy at the last challenge.
+    const TC r = ((const Ctl<FC>*)((const uint8_t*)host_ + o_ctl_))->r;
+    std::vector<TC> prev((size_t)K * 2 * len);
+    if (bound == 1) {
+      std::vector<TT> tmp((size_t)K * 2 * len);
+      for (uint32_t q = 0; q < K; ++q)
+        ZS_HIP(hipMemcpyAsync(tmp.data() + (size_t)q * 2 * len, args_.in + (size_t)q * args_.in_stride,
+                              2 * len * sizeof(TT), hipMemcpyDeviceToHost, s));
+      ZS_HIP(hipStreamSynchronize(s));
+      for (size_t i = 0; i < tmp.size(); ++i) prev[i] = Lift<FT, FC>::go(tmp[i]);
+    } else {
+      for (uint32_t q = 0; q < K; ++q)
+        ZS_HIP(hipMemcpyAsync(prev.data() + (size_t)q * 2 * len,
+                              args_.work + (size_t)q * args_.work_stride, 2 * len * sizeof(TC),
+                              hipMemcpyDeviceToHost, s));
+      ZS_HIP(hipStreamSynchronize(s));
+    }
+    TC* o = (TC*)out;
     for (uint32_t q = 0; q < K; ++q)
-      ZS_HIP(hipMemcpyAsync((TC*)out + (size_t)q * len, args_.work + (size_t)q * args_.work_stride,
-                            len * sizeof(TC), hipMemcpyDeviceToHost, s));
-    ZS_HIP(hipStreamSynchronize(s));
+      for (size_t j = 0; j < len; ++j) {
+        const TC lo = prev[(size_t)q * 2 * len + j], hi = prev[(size_t)q * 2 * len + len + j];
+        o[(size_t)q * len + j] = FC::add(lo, FC::mul(r, FC::sub(hi, lo)));
+      }
     return ZS_OK;
   }
 
@@ -483,6 +536,7 @@ class InstanceImpl final : public Instance {
  private:
   zs_ctx* ctx_ = nullptr;
   uint32_t n_ = 0;
+  uint32_t refold_ = 0;  // sqrt-space rounds before the dense ones
   TuneParams tune_;
   bool external_ = false;  // block_ and host_ belong to the caller
   void* block_ = nullptr;

@@ -115,6 +115,8 @@ struct StageArgs {
   const typename FC::T* eq_l_fam;  // suffix family of eq_point[0 : n/2]
   const typename FC::T* eq_r_fam;  // suffix family of eq_point[n/2 : n]
   uint64_t* trace;                 // phase timestamps (null: off)
+  typename FC::T* sq_eq;           // sqrt-space engine: two eq(r[<i], .) buffers of 2^sq_half
+  uint32_t sq_half;                // sqrt-space engine: refold rounds (0: none)
 };
 
 // Phase trace: trace[0] counts events; event i is (round << 8 | slot,

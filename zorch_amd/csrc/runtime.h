@@ -93,7 +93,8 @@ struct KernelRecord {
 enum {
   KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5,
   KREC_PASS_LA = 6, KREC_CHAIN_LA = 7, KREC_TAIL_LA = 8,  // look-ahead variants
-  KREC_FUSED01 = 9, KREC_FOLD2 = 10                       // rounds 0-2 from the input
+  KREC_FUSED01 = 9, KREC_FOLD2 = 10,                      // rounds 0-2 from the input
+  KREC_REFOLD = 11, KREC_ENGINE = 12                      // sqrt-space rounds; other engines
 };
 
 }  // namespace zs
@@ -278,6 +279,14 @@ inline uint32_t wire_width(const zs_wire* w, uint32_t npts, bool product_two) {
 int noalloc_scratch_init(int device);
 
 uint32_t field_bytes(uint32_t field);
+uint32_t base_of(uint32_t field);  // base prime field of a field id
+int set_device(zs_ctx* ctx);
+// One host Fiat-Shamir hop (observe under slice framing, sample `limbs`
+// words) on either transcript kind; false if a BLAKE3 stream outgrows its
+// subtree stack.
+bool transcript_hop(zs_transcript* t, const uint8_t* msg, size_t len, size_t item, size_t limbs,
+                    uint8_t* out);
+
 // Structural checks of a request: the natural value count npts and the
 // message width mlen on its wire form.
 int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts,

@@ -502,6 +502,8 @@ int zs_server_submit(zs_server* S, const zs_prove_args* args, const zs_transcrip
   uint32_t npts = 0, mlen = 0;
   int rc = validate_args(*args, t, &npts, &mlen);
   if (rc) return rc;
+  if (args->refold_rounds)
+    return fail(ZS_EINVAL, "served proofs fold every round (refold_rounds must be 0)");
   const bool eq = args->mode == ZS_MODE_EQ;
   const uint32_t cb = field_bytes(args->chal_field);
   Pending p;

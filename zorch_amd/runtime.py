@@ -194,12 +194,14 @@ class DeviceFoldState:
 
     Rounds advance it in place; ``bound`` counts the rounds done. A carry that
     refers to an earlier ``bound`` has been consumed and cannot be replayed.
+    With ``refold_rounds`` the first rounds leave the input unfolded and
+    refold it over the bound prefix (the sqrt-space engine).
     """
 
     def __init__(self, tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
                  eq_point: FieldArray | None = None, ctx: Context | None = None, *,
                  wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None,
-                 eq_mass: FieldArray | None = None):
+                 eq_mass: FieldArray | None = None, refold_rounds: int = 0):
         self.ctx = ctx or default_context()
         ptr, on_dev, keep, shape = tables_pointer(tables)
         self.k, width = shape
@@ -216,7 +218,7 @@ class DeviceFoldState:
         args = _lib.ProveArgsC(
             table_field=self.table_field.fid, chal_field=self.chal_field.fid, mode=mode,
             degree=degree, num_tables=self.k, rounds=self.rounds, tables=ptr,
-            tables_on_device=1 if on_dev else 0, reserved=0,
+            tables_on_device=1 if on_dev else 0, refold_rounds=refold_rounds,
             eq_point=eq_buf.ctypes.data if eq_buf is not None else None,
             claim=claim_buf.ctypes.data,
             wire=ctypes.pointer(wire) if wire is not None else None,

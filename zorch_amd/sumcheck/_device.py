@@ -49,9 +49,10 @@ class StageJob:
 
 def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
              eq_point: FieldArray | None = None, *, wire_kind: int = _lib.ZS_WIRE_VALUES,
-             domain: Any = None) -> StageJob:
+             domain: Any = None, refold_rounds: int = 0) -> StageJob:
     """Arguments and output buffers of one stage; `wire_kind` / `domain` pick
-    the message wire form (natural values by default)."""
+    the message wire form (natural values by default); `refold_rounds` first
+    rounds refold the untouched input (the sqrt-space engine)."""
     chal = as_field(chal_field)
     ptr, on_dev, keep, shape = tables_pointer(tables)
     k, width = shape
@@ -65,7 +66,8 @@ def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
     wire_ptr = ctypes.pointer(wire) if wire is not None else None
     args = _lib.ProveArgsC(
         table_field=as_field(table_field).fid, chal_field=chal.fid, mode=mode, degree=degree,
-        num_tables=k, rounds=n, tables=ptr, tables_on_device=1 if on_dev else 0, reserved=0,
+        num_tables=k, rounds=n, tables=ptr, tables_on_device=1 if on_dev else 0,
+        refold_rounds=refold_rounds,
         eq_point=eq_buf.ctypes.data if eq_buf is not None else None, claim=claim_buf.ctypes.data,
         wire=wire_ptr)
     npts = message_width(args)
@@ -88,10 +90,11 @@ def _result(job: StageJob, st: _lib.TranscriptC, transcript) -> DeviceProof:
 
 def prove_stage(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int, transcript,
                 eq_point: FieldArray | None = None, ctx: Context | None = None, *,
-                wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None) -> DeviceProof:
+                wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None,
+                refold_rounds: int = 0) -> DeviceProof:
     ctx = ctx or default_context()
     job = make_job(tables, claim, chal_field, mode, degree, eq_point, wire_kind=wire_kind,
-                   domain=domain)
+                   domain=domain, refold_rounds=refold_rounds)
     st = transcript.state.copy()
     _lib.check(_lib.lib().zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
                                    ctypes.byref(job.proof)))

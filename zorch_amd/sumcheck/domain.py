@@ -234,8 +234,107 @@ def product_round_coeffs(stacked) -> FieldArray:
     return _device_message(stacked, _lib.ZS_WIRE_COEFFS, None)
 
 
+# --- two-adic subgroup (univariate skip; reference domain.py:237-311) -------
+# Host helpers for small arrays (the device round 0 has its own transforms).
+# The subgroup of order 2^k is generated by g^((p-1)/2^k), g the field's
+# multiplicative generator (BabyBear 31, Goldilocks 7); the reference takes
+# its root from FRX's NTT, which is not visible here.
+
+_GENERATOR = {"babybear_mont": 31, "goldilocks": 7}
+
+
+def subgroup_root(dtype: Any, k: int) -> int:
+    """A primitive 2^k-th root of unity of the prime field under `dtype`
+    (canonical integer)."""
+    f = as_field(dtype).base
+    if (f.p - 1) % (1 << k):
+        raise ValueError(f"{f.name} has no subgroup of order 2^{k}")
+    return pow(_GENERATOR[f.name], (f.p - 1) >> k, f.p)
+
+
+def _dft_last(values: FieldArray, inverse: bool) -> FieldArray:
+    f = values.field
+    if f.degree != 1:
+        raise ValueError("subgroup transforms take base-field values")
+    n = values.shape[-1]
+    k = n.bit_length() - 1
+    if n & (n - 1):
+        raise ValueError(f"subgroup size {n} is not a power of two")
+    w = subgroup_root(f, k) if n > 1 else 1
+    if inverse:
+        w = pow(w, f.p - 2, f.p)
+    rows = [[int(x) for x in r] for r in values.canonical()[..., 0].reshape(-1, n).tolist()]
+    ninv = pow(n, f.p - 2, f.p) if inverse else 1
+    out = []
+    for r in rows:
+        row = []
+        for j in range(n):
+            wj, x, acc = pow(w, j, f.p), 1, 0
+            for v in r:
+                acc = (acc + v * x) % f.p
+                x = x * wj % f.p
+            row.append(acc * ninv % f.p)
+        out.append(row)
+    return array(out, f).reshape(values.shape)
+
+
+def subgroup_to_coeffs(values: FieldArray) -> FieldArray:
+    """Ascending coefficients of the degree < |D| univariate with `values` on
+    the order-|D| subgroup (last axis; the iNTT, domain.py:245-252)."""
+    return _dft_last(values, True)
+
+
+def subgroup_evals(coeffs: FieldArray, size: int) -> FieldArray:
+    """`coeffs` (last axis) evaluated on the order-`size` subgroup: zero-pad,
+    then the NTT (domain.py:255-266)."""
+    pad = coeffs.shape[-1]
+    if size < pad:
+        raise ValueError(f"subgroup_evals size {size} < coeff count {pad}")
+    from zorch_amd.fields import zeros
+    tail = zeros(coeffs.shape[:-1] + (size - pad,), coeffs.field)
+    return _dft_last(concatenate([coeffs, tail], axis=-1), False)
+
+
+def subgroup_sum(coeffs: FieldArray, skip_rounds: int) -> FieldArray:
+    """sum over the order-2^skip_rounds subgroup D of the univariate `coeffs`:
+    |D| times the sum of the coefficients at multiples of |D| (domain.py:
+    269-278)."""
+    size = 1 << skip_rounds
+    f = coeffs.field
+    acc = array(0, f)
+    for i in range(0, coeffs.shape[0], size):
+        acc = acc + coeffs[i]
+    return acc * array(size, f)
+
+
+@dataclass(frozen=True)
+class UnivariateSkipDomain:
+    """The order-2^skip_rounds subgroup D of the skip's round 0 (domain.py:
+    281-311), with the value/coefficient maps carried by the transforms."""
+
+    skip_rounds: int
+    dtype: Any
+
+    @property
+    def size(self) -> int:
+        return 1 << self.skip_rounds
+
+    def nodes(self) -> FieldArray:
+        """The |D| points in transform order: w^0, w^1, ..."""
+        f = as_field(self.dtype).base
+        w = subgroup_root(f, self.skip_rounds)
+        return array([pow(w, j, f.p) for j in range(self.size)], f)
+
+    def to_coeffs(self, values: FieldArray) -> FieldArray:
+        return subgroup_to_coeffs(values)
+
+    def sum_over_subgroup(self, coeffs: FieldArray) -> FieldArray:
+        return subgroup_sum(coeffs, self.skip_rounds)
+
+
 __all__ = [
     "EvalDomain", "natural_domain", "uhat_domain", "compressed_domain",
     "extend_to_round_domain", "split_halves", "fold", "interp_coeffs",
-    "product_round_poly", "product_round_coeffs",
+    "product_round_poly", "product_round_coeffs", "subgroup_root", "subgroup_to_coeffs",
+    "subgroup_evals", "subgroup_sum", "UnivariateSkipDomain",
 ]

@@ -74,3 +74,16 @@ def reduce_compressed(claim: FieldArray, msg: FieldArray, r: FieldArray):
     c0, c2 = msg[0], msg[1]
     c1 = claim - c0 - c0 - c2
     return eval_coeffs(stack([c0, c1, c2]), r), True
+
+
+def reduce_subgroup(claim: FieldArray, msg: FieldArray, r: FieldArray, skip_rounds: int,
+                    degree: int):
+    """The skip's round 0 (reduce.py:68-74): `msg` is s_0's ascending
+    coefficients, checked against the subgroup sum claim == sum_{z in D} s_0(z)
+    rather than the hypercube identity."""
+    from zorch_amd.sumcheck.domain import subgroup_sum
+    require_width(msg, degree * ((1 << skip_rounds) - 1) + 1, "coefficients")
+    ok = bool(claim.astype(msg.field) == subgroup_sum(msg, skip_rounds)) \
+        if claim.field.degree == 1 else bool(claim == subgroup_sum(msg, skip_rounds).astype(
+            claim.field))
+    return eval_coeffs(msg.astype(r.field), r), ok

@@ -107,3 +107,24 @@ class CompressedCoeffsSumcheckRound:
         transcript, r = self.challenges.observe_and_sample(transcript, msg)
         reduced, ok = self.check_reduce(value, msg, r)
         return claim.bind(reduced, r), transcript, ok
+
+
+@dataclass(frozen=True)
+class UnivariateSkipRound:
+    """Verifier of the skip's round 0 (reference verifier.py, UnivariateSkipRound;
+    univariate_skip.py:193-245): the base-field coefficient message is
+    observed, the challenge drawn in the challenge field, the subgroup-sum
+    identity checked and the claim reduced to s_0(r_0)."""
+
+    skip_rounds: int
+    degree: int
+    challenges: ChallengePolicy
+
+    def __call__(self, claim: RunningClaim, transcript, msg: FieldArray):
+        require_width(msg, self.degree * ((1 << self.skip_rounds) - 1) + 1, "coefficients")
+        if msg.ndim != 1:
+            raise ValueError("a round message is one row of the proof")
+        from zorch_amd.sumcheck.reduce import reduce_subgroup
+        transcript, r = self.challenges.observe_and_sample(transcript, msg)
+        reduced, ok = reduce_subgroup(claim.value, msg, r, self.skip_rounds, self.degree)
+        return claim.bind(reduced, r), transcript, ok
