@@ -229,6 +229,7 @@ class DeviceFoldState:
         self._h = h
         self._keep = keep
         self.bound = 0
+        self.start_index = 0  # point index of this state's first round in its claim
         self.eq_point = eq_point
 
     def round(self, transcript):
@@ -258,12 +259,15 @@ class DeviceFoldState:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None:
+        ctx = getattr(self, "ctx", None)
+        # A state whose context is already closed (finalisers of a cycle run in
+        # any order at shutdown) went with the context's memory.
+        if h is not None and ctx is not None and ctx._h is not None:
             try:
                 _lib.lib().zs_state_destroy(h)
             except Exception:
                 pass
-            self._h = None
+        self._h = None
 
 
 def profile(ctx: Context, enable: bool) -> None:

@@ -86,7 +86,10 @@ def _advance_device(carry: FoldingClaim, transcript, make_state) -> tuple:
     state = carry.state
     if not isinstance(state, DeviceFoldState):
         state = make_state(state)
-    if state.bound != carry.claim.index:
+        # A state made here starts from the claim's current index (a skip's
+        # bound prism starts after r0).
+        state.start_index = carry.claim.index
+    if state.start_index + state.bound != carry.claim.index:
         raise ValueError("this carry was already advanced; a device fold state is consumed "
                          "by each round")
     msg, chal, reduced, transcript = state.round(transcript)
