@@ -16,7 +16,9 @@ The metric has three parts, all in the one JSON line a default run prints:
   zerocheck) in one zs_prove_batch against 512 one-proof calls;
 * ``serve``: Poisson arrivals at 2000 requests/s over 60 s (zerocheck,
   n ~ U{14..24}) through the continuous-batching server, p50/p99 latency and
-  proofs/s, against one proof per call.
+  proofs/s, against one proof per call;
+* ``engines``: the sqrt-space, univariate-skip and small-value engines at
+  n = 24 against the dense prover on the same tables.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
                     [--no-batch] [--no-serve] [--no-cpu-baseline]
@@ -571,6 +573,98 @@ def bench_single(args, inst, peak: dict | None) -> dict:
     }
 
 
+def bench_engines(args, n: int = 24) -> dict:
+    """The reference's round-cost engines at n = 24 on the single proof's
+    tables (two BabyBear tables, BabyBear^4 challenges), each against the
+    dense prover on the same inputs and wire: sqrt-space and univariate skip
+    (product summand), small-value (eq-weighted; one transcript word a
+    challenge, so the transcript is over BabyBear^4). Whole proofs from the
+    host call to the returned proof, tables resident; the engine's own first
+    kernel timed with HIP events (refold rounds: bytes a round over time)."""
+    from zorch_amd import _lib
+    from zorch_amd import workloads as W
+    from zorch_amd.challenge import ChallengePolicy
+    from zorch_amd.fields import babybearx4_mont
+    from zorch_amd.runtime import DeviceTables, default_context, profile, profile_read
+    from zorch_amd.sumcheck._device import prove_stage
+    from zorch_amd.sumcheck.domain import uhat_domain
+    from zorch_amd.sumcheck.eq.small_value import prove_eq_poly_small_value
+    from zorch_amd.sumcheck.univariate_skip import prove_univariate_skip
+
+    ctx = default_context()
+    reps = max(3, min(args.steps, 10))
+    prod = W.product_bb_ext(n, (1, 2), 0)
+    zc = W.zerocheck_bb4(n, transcript_field=babybearx4_mont)
+    tables = DeviceTables.from_host(prod.tables, ctx)
+    uhat = uhat_domain(2, prod.chal)
+    base_claim = W.product_claim(prod.tables, prod.tables.field)
+    pol = ChallengePolicy(prod.chal)
+    elems = prod.table_elems
+
+    def timed(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    def profiled(fn):
+        profile(ctx, True)
+        fn()
+        recs = profile_read(ctx)
+        profile(ctx, False)
+        return recs
+
+    dense_u = lambda: prove_stage(tables, prod.claim, prod.chal, _lib.ZS_MODE_PRODUCT, 2,
+                                  W.new_transcript(prod), wire_kind=_lib.ZS_WIRE_DOMAIN,
+                                  domain=uhat)
+    sqrt_u = lambda: prove_stage(tables, prod.claim, prod.chal, _lib.ZS_MODE_PRODUCT, 2,
+                                 W.new_transcript(prod), wire_kind=_lib.ZS_WIRE_DOMAIN,
+                                 domain=uhat, refold_rounds=n // 2)
+    dense_v = lambda: prove_stage(tables, prod.claim, prod.chal, _lib.ZS_MODE_PRODUCT, 2,
+                                  W.new_transcript(prod))
+    out = {"n": n, "reps": reps, "tables": "two BabyBear tables of 2^24 (the single proof's)"}
+    same = dense_u().msgs.words.tolist() == sqrt_u().msgs.words.tolist()
+    ms_d, ms_s = timed(dense_u), timed(sqrt_u)
+    recs = [r for r in profiled(sqrt_u) if r["kind"] == "refold"]
+    refold_gbs = [r["bytes"] / (r["ms"] * 1e-3) / 1e9 for r in recs]
+    out["sqrt_space"] = {
+        "ms_per_proof": ms_s, "dense_ms_per_proof": ms_d, "table_elems_per_s": elems / ms_s * 1e3,
+        "identical_to_dense": same, "wire": "U-hat_2", "refold_rounds": n // 2,
+        "refold_round_ms": [r["ms"] for r in recs], "refold_round_gbs": refold_gbs,
+        "folded_state_bytes": 2 * 2 * (2 << (n - n // 2)) * 16,
+        "dense_folded_state_bytes": 2 * (1 << (n - 1)) * 16}
+    for skip in (3,):
+        skip_fn = lambda: prove_univariate_skip(tables, base_claim, skip, W.new_transcript(prod),
+                                                challenges=pol)
+        ms_k = timed(skip_fn)
+        recs = [r for r in profiled(skip_fn) if r["kind"] == "engine"]
+        out["univariate_skip"] = {
+            "skip_rounds": skip, "ms_per_proof": ms_k, "dense_ms_per_proof": timed(dense_v),
+            "table_elems_per_s": elems / ms_k * 1e3,
+            "round0_ms": recs[0]["ms"] if recs else None,
+            "round0_gbs": recs[0]["bytes"] / (recs[0]["ms"] * 1e-3) / 1e9 if recs else None,
+            "rounds_sent": 1 + n - skip}
+    tz = DeviceTables.from_host(zc.tables, ctx)
+    dense_e = lambda: prove_stage(tz, zc.claim, zc.chal, _lib.ZS_MODE_EQ, 2, W.new_transcript(zc),
+                                  zc.z, wire_kind=_lib.ZS_WIRE_DOMAIN, domain=uhat)
+    for l0 in (3,):
+        sv_fn = lambda: prove_eq_poly_small_value(tz, zc.z, l0, zc.claim, W.new_transcript(zc),
+                                                  challenges=ChallengePolicy(zc.chal))
+        same = [m.words.tolist() for m in sv_fn()[2]] == dense_e().msgs.words.tolist()
+        ms_v, ms_e = timed(sv_fn), timed(dense_e)
+        recs = [r for r in profiled(sv_fn) if r["kind"] == "engine"]
+        out["small_value"] = {
+            "l0": l0, "ms_per_proof": ms_v, "dense_ms_per_proof": ms_e,
+            "table_elems_per_s": elems / ms_v * 1e3, "identical_to_dense": same,
+            "accumulator_pass_ms": recs[0]["ms"] if recs else None,
+            "accumulator_pass_gbs": recs[0]["bytes"] / (recs[0]["ms"] * 1e-3) / 1e9
+            if recs else None, "transcript": "sha256 over babybearx4_mont"}
+    tables.free()
+    tz.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -580,6 +674,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="leave out the batched part")
     ap.add_argument("--no-serve", action="store_true", help="leave out the served part")
+    ap.add_argument("--no-engines", action="store_true",
+                    help="leave out the sqrt-space / univariate-skip / small-value part")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
@@ -631,6 +727,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(inst)
         print("cpu baseline done", file=sys.stderr, flush=True)
+    if full and not args.no_engines:
+        out["engines"] = bench_engines(args)
+        print("engines done", file=sys.stderr, flush=True)
     if batch_insts is not None:
         out["batch"] = bench_batch(args, batch_insts)
         del batch_insts
