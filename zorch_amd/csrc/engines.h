@@ -15,7 +15,10 @@ namespace zs {
 // L = 2^(n-i)) is refolded from them over the bound prefix,
 //   lo_q = sum_b E[b] P_q[b L + j],   hi_q = sum_b E[b] P_q[b L + L/2 + j],
 // with E = eq(r[<i], .) (2^i challenge-field entries, the newest challenge on
-// the index LSB; E_0 = [1]). A block takes groups of P consecutive pairs
+// the index LSB). Round 0 (E_0 = [1]) is the dense round 0 and round 1 forms
+// E_1 = [1 - r_0, r_0] from the challenge; later rounds read E from one of two
+// buffers, which block 0 extends by the round's challenge once the round is
+// finished. A block takes groups of P consecutive pairs
 // (coalesced) split over S = 256/P slices of the prefix: while pairs are many
 // P = 256 and a thread refolds whole pairs; as they grow few the prefix sum is
 // spread over the block and the slices meet in LDS. Slice-0 threads form the
@@ -46,6 +49,11 @@ __global__ __launch_bounds__(PASS_THREADS) void refold_pass_kernel(StageArgs<FT,
   const size_t half_buf = (size_t)1 << a.sq_half;
   const T* E = a.sq_eq + ((round & 1u) ? half_buf : 0);
   T* E_next = a.sq_eq + ((round & 1u) ? 0 : half_buf);
+  // E_1 = [1 - r_0, r_0], from the challenge of round 0 (read before this
+  // round's finisher replaces it).
+  const T r_prev = ctl->r;
+  const T e1_lo = FC::sub(ctl->one, r_prev);
+  auto e_at = [&](uint32_t b) -> T { return round == 1 ? (b ? r_prev : e1_lo) : E[b]; };
   if (blockIdx.x == 0) {
     if (tid < 64) {
       fin_load(sh.sm, ctl);
@@ -72,7 +80,7 @@ __global__ __launch_bounds__(PASS_THREADS) void refold_pass_kernel(StageArgs<FT,
       const T r = sh.sm.r;
       const uint32_t B = 1u << round;
       for (uint32_t x = tid; x < 2 * B; x += PASS_THREADS) {
-        const T e = round == 0 ? ctl->one : E[x >> 1];
+        const T e = e_at(x >> 1);
         const T hi = FC::mul(e, r);
         E_next[x] = (x & 1u) ? hi : FC::sub(e, hi);
       }
@@ -85,7 +93,6 @@ __global__ __launch_bounds__(PASS_THREADS) void refold_pass_kernel(StageArgs<FT,
   const uint32_t pj = tid & (P - 1), slice = tid >> pshift;
   const uint32_t B = 1u << round;
   const uint32_t groups = half >> pshift;
-  const T one = ctl->one;
   T acc[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) acc[t] = FC::zero();
@@ -97,8 +104,30 @@ __global__ __launch_bounds__(PASS_THREADS) void refold_pass_kernel(StageArgs<FT,
       lo[q].init();
       hi[q].init();
     }
-    for (uint32_t b = slice; b < B; b += S) {
-      const T e = round == 0 ? one : E[b];
+    // Two prefix rows a step: the lazy sums take two products between folds.
+    uint32_t b = slice;
+    for (; b + S < B; b += 2 * S) {
+      const T e0 = e_at(b), e1 = e_at(b + S);
+      const size_t base0 = (size_t)b * L + j, base1 = (size_t)(b + S) * L + j;
+      typename FT::T x[K][4];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const typename FT::T* tq = a.in + (size_t)q * a.in_stride;
+        x[q][0] = tq[base0];
+        x[q][1] = tq[base0 + half];
+        x[q][2] = tq[base1];
+        x[q][3] = tq[base1 + half];
+      }
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        lo[q].add(e0, x[q][0], false);
+        hi[q].add(e0, x[q][1], false);
+        lo[q].add(e1, x[q][2]);
+        hi[q].add(e1, x[q][3]);
+      }
+    }
+    if (b < B) {
+      const T e = e_at(b);
       const size_t base = (size_t)b * L + j;
 #pragma unroll
       for (int q = 0; q < K; ++q) {

@@ -141,6 +141,7 @@ static int prove_tail(zs_ctx* ctx, const zs_prove_args& base, uint32_t chal_fiel
 // The host interpolates the M sums back to the message's coefficients. All
 // base-field arithmetic.
 constexpr uint32_t SKIP_MAX_W = 4096;   // M x D matrix entries
+constexpr uint32_t SKIP_TILE = 2048;    // K x D x (256 / M) column values of a step
 constexpr uint32_t SKIP_ROW = 512;      // 64-bit words of a slot row (M <= 256, 2 words)
 
 template <class FT, int K>
@@ -151,26 +152,39 @@ __global__ __launch_bounds__(256) void skip_round0_kernel(const typename FT::T* 
                                                           uint64_t* slots) {
   using T = typename FT::T;
   __shared__ T w[SKIP_MAX_W];
+  __shared__ T tile[SKIP_TILE];  // [K][D][cols] column values of the current step
   __shared__ T red[256];
   const uint32_t tid = threadIdx.x;
   const uint32_t D = 1u << dlog, M = 1u << mlog;
   for (uint32_t i = tid; i < (M << dlog); i += 256) w[i] = wmat[i];
-  __syncthreads();
   const uint32_t m = tid & (M - 1), c = tid >> mlog;
-  const uint32_t cols = 256u >> mlog;
+  const uint32_t clog = 8u - mlog, cols = 1u << clog;
   const size_t hn = (size_t)1 << hn_log;
   const T* wm = w + ((size_t)m << dlog);
+  const uint32_t nt = (uint32_t)K << (dlog + clog);  // tile entries
   T acc = FT::zero();
-  for (size_t x = (size_t)blockIdx.x * cols + c; x < hn; x += (size_t)gridDim.x * cols) {
-    T prod = FT::one();
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      const T* col = in + (size_t)q * in_stride + x;
-      T e = FT::zero();
-      for (uint32_t b = 0; b < D; ++b) e = FT::add(e, FT::mul(wm[b], col[(size_t)b << hn_log]));
-      prod = q == 0 ? e : FT::mul(prod, e);
+  // Every thread runs every step (the tile load is shared): the steps are
+  // the block's column groups, grid-strided.
+  for (size_t x0 = (size_t)blockIdx.x * cols; x0 < hn; x0 += (size_t)gridDim.x * cols) {
+    __syncthreads();
+    for (uint32_t i = tid; i < nt; i += 256) {
+      const uint32_t cc = i & (cols - 1), qb = i >> clog;  // qb = q D + b
+      const uint32_t q = qb >> dlog, b = qb & (D - 1);
+      const size_t x = x0 + cc;
+      tile[i] = x < hn ? in[(size_t)q * in_stride + ((size_t)b << hn_log) + x] : FT::zero();
     }
-    acc = FT::add(acc, prod);
+    __syncthreads();
+    if (x0 + c < hn) {
+      T prod = FT::one();
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const T* col = tile + ((size_t)q << (dlog + clog)) + c;
+        T e = FT::zero();
+        for (uint32_t b = 0; b < D; ++b) e = FT::add(e, FT::mul(wm[b], col[b << clog]));
+        prod = q == 0 ? e : FT::mul(prod, e);
+      }
+      acc = FT::add(acc, prod);
+    }
   }
   red[tid] = acc;
   __syncthreads();
@@ -221,7 +235,7 @@ int skip_bind_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t skip, zs_trans
   uint32_t mlog = 0;
   while ((1u << mlog) < d0 + 1) ++mlog;
   const uint32_t M = 1u << mlog;
-  if (M > 256 || (size_t)M * D > SKIP_MAX_W)
+  if (M > 256 || (size_t)M * D > SKIP_MAX_W || (size_t)K * D * (256 / M) > SKIP_TILE)
     return fail(ZS_EINVAL, "skip_rounds too large: the round-0 message needs " +
                                std::to_string(M) + " subgroup values (at most 256, and " +
                                "at most 4096 extension-matrix entries)");
@@ -412,46 +426,70 @@ int skip_dispatch_k(zs_ctx* ctx, const zs_prove_args& a, uint32_t skip, zs_trans
 // stride is a multiple of 2^lo_bits), so eq_lo is applied once at the end.
 // The host contracts T into the accumulators and runs the l0 rounds.
 constexpr uint32_t SV_MAX_C = 8192;  // nbeta x 2^l0 matrix entries
+constexpr uint32_t SV_TILE = 1024;   // K x 2^l0 x (256 / 2^blog) column values of a step
+
+// The prime field under a table field: the accumulator pass's integer
+// coefficients are stored there.
+template <class F>
+struct BaseOf {
+  using type = F;
+};
+template <>
+struct BaseOf<BB4> {
+  using type = BB;
+};
 constexpr uint32_t SV_ROW = 1024;    // 64-bit words of a slot row (256 betas, 4 words)
 
 template <class FT, class FC, int K>
 __global__ __launch_bounds__(256) void sv_accum_kernel(const typename FT::T* in, size_t in_stride,
                                                        uint32_t l0, uint32_t slog, uint32_t lo_bits,
                                                        uint32_t blog, uint32_t nbeta,
-                                                       const typename FT::T* cmat,
+                                                       const typename BaseOf<FT>::type::T* cmat,
                                                        const typename FC::T* eq_hi,
                                                        const typename FC::T* eq_lo,
                                                        uint64_t* slots) {
   using TT = typename FT::T;
   using TC = typename FC::T;
-  __shared__ TT cm[SV_MAX_C];
+  using FB = typename BaseOf<FT>::type;
+  __shared__ typename FB::T cm[SV_MAX_C];
+  __shared__ TT tile[SV_TILE];  // [K][2^l0][cols] column values of the current step
   __shared__ TC red[256];
   const uint32_t tid = threadIdx.x;
   const uint32_t P = 1u << l0;
   for (uint32_t i = tid; i < nbeta * P; i += 256) cm[i] = cmat[i];
-  __syncthreads();
   const uint32_t beta = tid & ((1u << blog) - 1), c = tid >> blog;
-  const uint32_t cols = 256u >> blog;
+  const uint32_t clog = 8u - blog, cols = 1u << clog;
   const size_t hn = (size_t)1 << slog;
-  const size_t x0 = (size_t)blockIdx.x * cols + c;
+  const size_t x0s = (size_t)blockIdx.x * cols;
   const bool live = beta < nbeta;
-  const TT* cb = cm + (size_t)(live ? beta : 0) * P;
+  const typename FB::T* cb = cm + (size_t)(live ? beta : 0) * P;
+  const uint32_t nt = (uint32_t)K << (l0 + clog);
   WeightedSum<FC, FT> acc;
   acc.init();
-  if (live) {
-    for (size_t x = x0; x < hn; x += (size_t)gridDim.x * cols) {
+  for (size_t x0 = x0s; x0 < hn; x0 += (size_t)gridDim.x * cols) {
+    __syncthreads();
+    for (uint32_t i = tid; i < nt; i += 256) {
+      const uint32_t cc = i & (cols - 1), qb = i >> clog;
+      const uint32_t q = qb >> l0, b = qb & (P - 1);
+      const size_t x = x0 + cc;
+      tile[i] = x < hn ? in[(size_t)q * in_stride + ((size_t)b << slog) + x] : FT::zero();
+    }
+    __syncthreads();
+    const size_t x = x0 + c;
+    if (live && x < hn) {
       TT prod = FT::one();
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        const TT* col = in + (size_t)q * in_stride + x;
+        const TT* col = tile + ((size_t)q << (l0 + clog)) + c;
         TT e = FT::zero();
-        for (uint32_t b = 0; b < P; ++b) e = FT::add(e, FT::mul(cb[b], col[(size_t)b << slog]));
+        for (uint32_t b = 0; b < P; ++b)
+          e = FT::add(e, MixMul<FT, FB>::go(col[b << clog], cb[b]));
         prod = q == 0 ? e : FT::mul(prod, e);
       }
       acc.add(eq_hi[x >> lo_bits], prod);
     }
   }
-  red[tid] = FC::mul(acc.get(), eq_lo[x0 & (((size_t)1 << lo_bits) - 1)]);
+  red[tid] = FC::mul(acc.get(), eq_lo[(x0s + c) & (((size_t)1 << lo_bits) - 1)]);
   __syncthreads();
   if (tid < nbeta) {
     TC s = FC::zero();
@@ -499,7 +537,8 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
   uint32_t blog = 0;
   while ((1u << blog) < nbeta) ++blog;
   const uint32_t P = 1u << l0;
-  if (nbeta > 256 || (size_t)nbeta * P > SV_MAX_C)
+  if (nbeta > 256 || (size_t)nbeta * P > SV_MAX_C ||
+      ((size_t)K * P << (8 - blog)) > SV_TILE)
     return fail(ZS_EINVAL, "l_0 too large for the accumulator pass: (degree + 1)^l_0 = " +
                                std::to_string(nbeta) + " (at most 256)");
   std::vector<TC> w(n);
@@ -507,7 +546,8 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
   TC claim = load_host_el<FC>(a.claim, 0);
   // cmat[beta][b]: the integer weights of the prefix values (MSB-first digits
   // of beta over U_d, MSB-first bits of b).
-  std::vector<TT> cmat((size_t)nbeta * P);
+  using TB = typename BaseOf<FT>::type::T;
+  std::vector<TB> cmat((size_t)nbeta * P);
   for (uint32_t be = 0; be < nbeta; ++be) {
     uint32_t digits[16];
     uint32_t v = be;
@@ -523,7 +563,7 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
         const int64_t f = digits[i] == 0 ? (bit ? 1 : -1) : (bit ? tnode : 1 - tnode);
         cf *= f;
       }
-      cmat[(size_t)be * P + bb] = FT::from_int(cf);
+      cmat[(size_t)be * P + bb] = BaseOf<FT>::type::from_int(cf);
     }
   }
   // eq(w[l0:], x) = eq_hi(x_hi) eq_lo(x_lo) with lo_bits low bits fixed per
@@ -543,7 +583,7 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
   const size_t o_in = a.tables_on_device ? 0 : take((size_t)K * N * sizeof(TT));
-  const size_t o_c = take(cmat.size() * sizeof(TT));
+  const size_t o_c = take(cmat.size() * sizeof(TB));
   const size_t o_hi = take(eq_hi.size() * sizeof(TC));
   const size_t o_lo = take(eq_lo.size() * sizeof(TC));
   const size_t o_slots = take((size_t)NSLOT * SV_ROW * sizeof(uint64_t));
@@ -556,7 +596,7 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
   const TT* in = a.tables_on_device ? (const TT*)a.tables : (const TT*)(b + o_in);
   if (!a.tables_on_device)
     ZS_HIP(hipMemcpyAsync(b + o_in, a.tables, (size_t)K * N * sizeof(TT), hipMemcpyHostToDevice, s));
-  ZS_HIP(hipMemcpyAsync(b + o_c, cmat.data(), cmat.size() * sizeof(TT), hipMemcpyHostToDevice, s));
+  ZS_HIP(hipMemcpyAsync(b + o_c, cmat.data(), cmat.size() * sizeof(TB), hipMemcpyHostToDevice, s));
   ZS_HIP(hipMemcpyAsync(b + o_hi, eq_hi.data(), eq_hi.size() * sizeof(TC), hipMemcpyHostToDevice, s));
   ZS_HIP(hipMemcpyAsync(b + o_lo, eq_lo.data(), eq_lo.size() * sizeof(TC), hipMemcpyHostToDevice, s));
   ZS_HIP(hipMemsetAsync(b + o_slots, 0, (size_t)NSLOT * SV_ROW * sizeof(uint64_t), s));
@@ -566,7 +606,7 @@ int sv_prove_typed(zs_ctx* ctx, const zs_prove_args& a, uint32_t l0, zs_transcri
     (void)hipEventRecord(ev, s);
   }
   sv_accum_kernel<FT, FC, K><<<(uint32_t)grid, 256, 0, s>>>(
-      in, N, l0, slog, lo_bits, blog, nbeta, (const TT*)(b + o_c), (const TC*)(b + o_hi),
+      in, N, l0, slog, lo_bits, blog, nbeta, (const TB*)(b + o_c), (const TC*)(b + o_hi),
       (const TC*)(b + o_lo), (uint64_t*)(b + o_slots));
   ZS_HIP(hipGetLastError());
   if (ev) {

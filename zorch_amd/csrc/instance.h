@@ -261,6 +261,16 @@ class InstanceImpl final : public Instance {
       const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
       for (uint32_t r = first; r < stop; ++r) {
         const size_t pairs = (size_t)1 << (n_ - r - 1);
+        if (r == 0) {
+          // Nothing is bound yet: round 0 is the dense round 0.
+          size_t grid = pass_grid(pairs);
+          hipEvent_t ev = begin(s);
+          pass_kernel<FT, FT, FC, K, EQ, false><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(
+              args_, args_.in, args_.in_stride, 1u << n_, 0, last == 1 ? 1u : 0u);
+          ZS_HIP(hipGetLastError());
+          if (ev) mark(s, 0, KREC_PASS, (uint32_t)grid, pass_bytes(0), ev);
+          continue;
+        }
         const uint32_t ps = refold_pshift(pairs, maxg);
         size_t grid = pairs >> ps;
         if (grid > maxg) grid = maxg;
