@@ -57,33 +57,51 @@ constexpr uint32_t HOP_MAX_BLOCKS = 5;
 constexpr uint32_t HOP_MAX_MSG = 96;
 
 // Lanes b < nblk expand block first + b (64 bytes at blocks + 64 (first + b))
-// into wk[64 (first + b) + t] = W_t + K_t, t = 0..63.
+// into wk[64 (first + b) + t] = W_t + K_t, t = 0..63. The expansion runs as a
+// loop of sixteen-step bodies (the sixteen live words stay in fixed
+// registers), which keeps the hop's code small enough to stay in the
+// instruction cache beside the other waves' code.
 __device__ __forceinline__ void ws_schedule(const uint8_t* blocks, uint32_t first, uint32_t nblk,
                                             uint32_t* wk) {
   const uint32_t lane = threadIdx.x & 63;
   if (lane < nblk) {
     const uint32_t b = first + lane;
-    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(blocks + 64u * b);
-    uint32_t* o = wk + 64u * b;
+    const uint4* b128 = reinterpret_cast<const uint4*>(blocks + 64u * b);
+    uint4* o4 = reinterpret_cast<uint4*>(wk + 64u * b);
     uint32_t w[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) w[t] = __builtin_bswap32(b32[t]);  // all loads before any store
+    for (int v = 0; v < 4; ++v) {  // all loads before any store
+      const uint4 x = b128[v];
+      w[4 * v] = __builtin_bswap32(x.x);
+      w[4 * v + 1] = __builtin_bswap32(x.y);
+      w[4 * v + 2] = __builtin_bswap32(x.z);
+      w[4 * v + 3] = __builtin_bswap32(x.w);
+    }
 #pragma unroll
-    for (int t = 0; t < 16; ++t) o[t] = w[t] + ZS_K256[t];
+    for (int v = 0; v < 4; ++v)
+      o4[v] = make_uint4(w[4 * v] + ZS_K256[4 * v], w[4 * v + 1] + ZS_K256[4 * v + 1],
+                         w[4 * v + 2] + ZS_K256[4 * v + 2], w[4 * v + 3] + ZS_K256[4 * v + 3]);
+#pragma unroll 1
+    for (int t0 = 16; t0 < 64; t0 += 16) {
 #pragma unroll
-    for (int t = 16; t < 64; ++t) {
-      const uint32_t x = w[(t - 15) & 15], y = w[(t - 2) & 15];
-      const uint32_t s0 = sha_xor3(sha_rotr(x, 7), sha_rotr(x, 18), x >> 3);
-      const uint32_t s1 = sha_xor3(sha_rotr(y, 17), sha_rotr(y, 19), y >> 10);
-      const uint32_t wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
-      w[t & 15] = wt;
-      o[t] = wt + ZS_K256[t];
+      for (int u = 0; u < 16; ++u) {
+        const uint32_t x = w[(u + 1) & 15], y = w[(u + 14) & 15];
+        const uint32_t s0 = sha_xor3(sha_rotr(x, 7), sha_rotr(x, 18), x >> 3);
+        const uint32_t s1 = sha_xor3(sha_rotr(y, 17), sha_rotr(y, 19), y >> 10);
+        w[u] = w[u] + s0 + w[(u + 9) & 15] + s1;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        o4[(t0 >> 2) + v] =
+            make_uint4(w[4 * v] + ZS_K256[t0 + 4 * v], w[4 * v + 1] + ZS_K256[t0 + 4 * v + 1],
+                       w[4 * v + 2] + ZS_K256[t0 + 4 * v + 2], w[4 * v + 3] + ZS_K256[t0 + 4 * v + 3]);
     }
   }
   wave_sync();
 }
 
-// The 64 rounds of one compression from an expanded schedule (LDS).
+// The 64 rounds of one compression from an expanded schedule (LDS), fully
+// unrolled (the W + K words read four at a time ahead of their rounds).
 __device__ __forceinline__ void sha256_rounds(uint32_t h[8], const uint32_t* wk) {
   uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
   const uint4* q = reinterpret_cast<const uint4*>(wk);
@@ -106,8 +124,8 @@ __device__ __forceinline__ void sha256_rounds(uint32_t h[8], const uint32_t* wk)
 
 // observe(msg) + sample(limbs), with limbs * item <= 32 (one squeeze counter).
 // `msg` (LDS, msg_len <= HOP_MAX_MSG bytes) is framed into `buf` (LDS,
-// >= msg_len + 20 bytes); `blocks` (LDS, HOP_MAX_BLOCKS x 64 B) and `wk`
-// (LDS, HOP_MAX_BLOCKS x 64 words, 16-byte aligned) hold the hop's blocks and
+// >= msg_len + 20 bytes); `blocks` (LDS, HOP_MAX_BLOCKS x 64 B, 16-byte
+// aligned) and `wk` (LDS, HOP_MAX_BLOCKS x 64 words, 16-byte aligned) hold the hop's blocks and
 // schedules, `dgw` (LDS, 8 words) the digest; the squeezed bytes land in
 // `out` (LDS). The stream's pending bytes live in `pend` (LDS, 64 B).
 __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* buf,
@@ -117,6 +135,19 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t count = msg_len / item;
   const uint32_t framed = msg_len + 20u;
+  // Absorb stream = pending || framed: `na` full blocks, then `rem` bytes
+  // pending; the squeeze tail block(s) start with those rem bytes.
+  const uint32_t nbytes = item * limbs;
+  const uint32_t plen = s.plen;
+  const uint32_t total = plen + framed;
+  const uint32_t na = total / 64u, rem = total % 64u;
+  const uint64_t absorbed = s.total + framed;
+  const uint32_t ntail = (rem + 8u + 9u <= 64u) ? 1u : 2u;
+  const uint32_t nfinal = (rem + nbytes >= 64u) ? 1u : 0u;
+  const uint64_t bits = (absorbed + 8u) * 8u;  // stream || le64(ctr = 0)
+  const uint32_t tail_end = 64u * (na + ntail);
+  // The framed message, observe(msg) then the squeeze request:
+  //   [OP_OBSERVE, KIND_SLICE] le64(count) msg [OP_SQUEEZE, KIND_SLICE] le64(limbs).
   for (uint32_t pos = lane; pos < framed; pos += 64u) {
     uint32_t b;
     if (pos == 0) b = OP_OBSERVE;
@@ -129,17 +160,8 @@ __device__ __forceinline__ void ws_fs_hop(WaveSha& s, uint8_t* pend, uint8_t* bu
     buf[pos] = (uint8_t)b;
   }
   wave_sync();
-  // Absorb stream = pending || framed: `na` full blocks, then `rem` bytes
-  // pending; the squeeze tail block(s) start with those rem bytes.
-  const uint32_t nbytes = item * limbs;
-  const uint32_t plen = s.plen;
-  const uint32_t total = plen + framed;
-  const uint32_t na = total / 64u, rem = total % 64u;
-  const uint64_t absorbed = s.total + framed;
-  const uint32_t ntail = (rem + 8u + 9u <= 64u) ? 1u : 2u;
-  const uint32_t nfinal = (rem + nbytes >= 64u) ? 1u : 0u;
-  const uint64_t bits = (absorbed + 8u) * 8u;  // stream || le64(ctr = 0)
-  const uint32_t tail_end = 64u * (na + ntail);
+  // The blocks: pending || framed, then the squeeze tail (le64(0) counter,
+  // 0x80, zeros, the bit length).
   for (uint32_t pos = lane; pos < tail_end; pos += 64u) {
     uint32_t b;
     if (pos < total) {

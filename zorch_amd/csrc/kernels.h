@@ -637,13 +637,21 @@ struct HopLds {
 
 #define ZS_LDS __attribute__((address_space(3)))
 
+// The BLAKE3 hop out of line: its byte-serial stream code (every lane runs
+// the same code on the LDS state) is large, and kept apart from the SHA-256
+// path so that the latter stays small in the instruction cache.
+static __device__ __attribute__((noinline)) void b3_fs_hop_ool(Blake3State* st, const uint8_t* m,
+                                                               uint32_t msg_len, uint32_t item,
+                                                               uint32_t limbs, uint8_t* out) {
+  b3_fs_hop(*st, m, msg_len, item, limbs, out);
+}
+
 // observe(msg) + sample(limbs) on the transcript state in `s`; the squeezed
 // bytes land in s.out. Wave 0, all lanes.
 __device__ __forceinline__ void fs_hop_body(HopLds& s, const uint8_t* m, uint32_t msg_len,
                                             uint32_t item, uint32_t limbs, uint32_t kind) {
   if (kind == TR_BLAKE3) {
-    // Every lane runs the same byte-serial stream code on the LDS state.
-    b3_fs_hop(s.b3, m, msg_len, item, limbs, s.out);
+    b3_fs_hop_ool(&s.b3, m, msg_len, item, limbs, s.out);
     wave_sync();
     return;
   }
@@ -695,6 +703,9 @@ struct FinishLds : HopLds {
   uint32_t wire, mlen, ncoef, claim_every;
   typename FC::T wmat[MAXPTS][MAXPTS], cmat[MAXPTS][MAXPTS], cclaim[MAXPTS];
   alignas(16) typename FC::T snat[MAXPTS];
+  // Every round's eq line coefficients, for finishers that run many rounds
+  // (fin_eq_table): a round reads them from LDS instead of memory.
+  typename FC::T eqab[2 * 32];
 };
 
 // Control block -> finisher LDS (wave 0, all lanes).
@@ -747,6 +758,26 @@ __device__ __forceinline__ void fin_eq(FinishLds<FC>& sm, const StageArgs<FS, FC
   if (EQ && (threadIdx.x & 63) == 0) {
     sm.eqA = a.eq_ab[2 * round];
     sm.eqB = a.eq_ab[2 * round + 1];
+  }
+}
+
+// Every round's eq line coefficients -> finisher LDS (wave 0, all lanes), for
+// fin_eq_lds.
+template <class FS, class FC, bool EQ>
+__device__ __forceinline__ void fin_eq_table(FinishLds<FC>& sm, const StageArgs<FS, FC>& a) {
+  if constexpr (EQ) {
+    const uint32_t lane = threadIdx.x & 63;
+    if (lane < 2 * a.n) sm.eqab[lane] = a.eq_ab[lane];
+    wave_sync();
+  }
+}
+
+// The eq line coefficients of `round` from the table fin_eq_table loaded.
+template <class FC, bool EQ>
+__device__ __forceinline__ void fin_eq_lds(FinishLds<FC>& sm, uint32_t round) {
+  if (EQ && (threadIdx.x & 63) == 0) {
+    sm.eqA = sm.eqab[2 * round];
+    sm.eqB = sm.eqab[2 * round + 1];
   }
 }
 
@@ -1752,7 +1783,10 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   __shared__ TailLaLds<FC, K> sh;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t n = a.n;
-  if (tid < 64) fin_load(sh.sm, a.ctl);
+  if (tid < 64) {
+    fin_load(sh.sm, a.ctl);
+    fin_eq_table<FS, FC, EQ>(sh.sm, a);
+  }
   if (tid == 0) {
     sh.go = first + 1;
     sh.arrived = 0;
@@ -1820,7 +1854,7 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
   bool ok = true;
   for (uint32_t k = first; k < n && ok; ++k) {
     if (lane == 0) trace_mark(a.trace, k, 1);
-    fin_eq<FS, FC, EQ>(sh.sm, a, k);
+    fin_eq_lds<FC, EQ>(sh.sm, k);
     if (k == first && cont) {
       // Round first's node sums: the previous launch's coefficients at
       // r_{first-1} (complete at the launch boundary: plain loads, all rows
@@ -2017,11 +2051,14 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
   const uint32_t nb0 = la_blocks(cont ? la_units(n, first) : 1u << (n - first - 2), G);
   if (blockIdx.x == 0) {
     // The finisher.
-    if (tid < 64) fin_load(sh.sm, ctl);
+    if (tid < 64) {
+      fin_load(sh.sm, ctl);
+      fin_eq_table<FT, FC, EQ>(sh.sm, a);
+    }
     if (tid == 0) trace_mark(a.trace, first, 7);
     uint32_t target = cont ? 0u : nb0;
     for (uint32_t k = first; k < last; ++k) {
-      if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, k);
+      if (tid < 64) fin_eq_lds<FC, EQ>(sh.sm, k);
       if (k == first && !cont) {
         if (tid == 0) sh.ok = chain_wait(&ctl->counter, target, &ctl->fault) ? 1u : 0u;
         __syncthreads();
