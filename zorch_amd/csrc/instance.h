@@ -293,7 +293,8 @@ class InstanceImpl final : public Instance {
     if (fused_first_rounds(first, last)) {
       const size_t N = (size_t)1 << n_;
       const uint64_t in_bytes = (uint64_t)K * N * sizeof(TT);
-      size_t grid = pass_grid(N / 4);
+      // A thread unit of fused01_kernel is a 16-byte vector of groups.
+      size_t grid = pass_grid(N / 4 / (16 / sizeof(TT)));
       hipEvent_t ev = begin(s);
       fused01_kernel<FT, FC, K, EQ><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(args_);
       ZS_HIP(hipGetLastError());

@@ -2322,125 +2322,222 @@ __device__ __forceinline__ typename FC::T eq_w(const typename FC::T* wl, const t
   return wr[j];
 }
 
-// fused01_kernel's streaming loop: groups g = start + i stride < N/4, one a
-// step (the nine lazy coefficient sums fill the registers), the next step's entries (and, with FAC, its eq weights: wl alone, the
-// per-thread wr factor applied by the caller) loaded before this step's
-// arithmetic. Group g: input entries g + m N/4 (m = 0..3) of every table,
-// i.e. round 0's pairs g and g + N/4 and round 1's pair g.
+// fused01_kernel's streaming loop: groups g = start + i stride < N/4, U a
+// step, the next step's entries (and, with FAC, its eq weights: wl alone,
+// the per-thread wr factor applied by the caller) loaded before this step's
+// arithmetic. Group g: input entries g + m N/4 (m = 0..3) of every table. Its
+// four entries of table q span the (x_0, x_1) square of round 1's pair g, so
+// round 1's factor at point t folded at r_0 = s is
+//   T_q(s, t) = A_q + s B_q,  A_q = x0 + t (x1 - x0),  B_q = (x2 - x0) + t ((x3 - x1) - (x2 - x0)),
+// and csum[t][i] sums the coefficient of s^i of prod_q T_q(s, t) weighted by
+// round 1's eq weight of pair g. Round 0's node sums follow from these: its
+// pair (x_1, g) has eq weight eq(w_1, x_1) times round 1's, so
+//   round0(s) = sum_i s^i (eq(w_1, 0) csum[0][i] + eq(w_1, 1) csum[1][i])
+// (both weights 1 without eq), which the finisher forms. Groups a step: as
+// many as keep the entries of two steps and the (K+1)^2 sums in registers.
+template <int K>
+struct FusedU {
+  static constexpr int value = K <= 2 ? 4 : K == 3 ? 2 : 1;
+};
 template <class FT, class FC, int K, bool EQ, bool FAC>
-__device__ __forceinline__ void fused01_stream(
-    const StageArgs<FT, FC>& a, size_t start, size_t stride, const typename FC::T* wl0,
-    const typename FC::T* wr0, uint32_t x0, const typename FC::T* wl1,
-    const typename FC::T* wr1, uint32_t x1, FusedSum<FC, FT, EQ> (&dsum)[K + 1],
-    FusedSum<FC, FT, EQ> (&csum)[K + 1][K + 1]) {
+__device__ __forceinline__ void fused01_stream(const StageArgs<FT, FC>& a, uint32_t start,
+                                               uint32_t stride, const typename FC::T* wl1,
+                                               const typename FC::T* wr1, uint32_t x1,
+                                               FusedSum<FC, FT, EQ> (&csum)[K + 1][K + 1]) {
   using T = typename FC::T;
   using S = typename FT::T;
   constexpr int NA = K + 1;
-  constexpr int U = 1;
+  constexpr int U = FusedU<K>::value;
   constexpr bool PW = EQ && FAC;  // weights prefetched with the entries
   constexpr int UW = PW ? U : 1;
-  const size_t q4 = (size_t)1 << (a.n - 2);
-  const size_t step = (size_t)U * stride;
-  auto load = [&](size_t g0, S (&x)[U][K][4], T (&wa)[UW], T (&wb)[UW], T (&wc)[UW]) {
+  // Group indices fit 32 bits, and so do their byte offsets into a quarter
+  // table (n <= 31): each stream is a uniform base and a 32-bit byte offset.
+  const uint32_t q4 = 1u << (a.n - 2);
+  const char* base[K][4];
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      base[q][m] = reinterpret_cast<const char*>(a.in + (size_t)q * a.in_stride + (size_t)m * q4);
+  const uint32_t step = (uint32_t)U * stride;
+  // Loads of the U groups from g0 (indices past the round clamped into it,
+  // so that every step issues the same loads).
+  auto load = [&](uint32_t g0, S (&x)[U][K][4], T (&wc)[UW]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      size_t g = g0 + (size_t)u * stride;
-      g = g < q4 ? g : q4 - 1;
+      const uint32_t g = min(g0 + (uint32_t)u * stride, q4 - 1u);
+      const uint32_t off = g * (uint32_t)sizeof(S);
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
-        const S* s = a.in + (size_t)q * a.in_stride;
+      for (int q = 0; q < K; ++q)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) x[u][q][m] = s[g + (size_t)m * q4];
-      }
-      if constexpr (PW) {
-        wa[u] = wl0[g >> x0];
-        wb[u] = wl0[(g + q4) >> x0];
-        wc[u] = wl1[g >> x1];
-      }
+        for (int m = 0; m < 4; ++m) x[u][q][m] = *reinterpret_cast<const S*>(base[q][m] + off);
+      if constexpr (PW) wc[u] = wl1[g >> x1];
     }
   };
-  S cur[U][K][4];
-  T ca[UW], cb[UW], cc[UW];
-  if (start < q4) load(start, cur, ca, cb, cc);
-  uint32_t par = 0;
-  for (size_t g0 = start; g0 < q4; g0 += step) {
-    S nxt[U][K][4];
-    T na[UW], nb[UW], nc[UW];
-    load(g0 + step, nxt, na, nb, nc);
+  // Group g from its entries e with weight w; `fold`: whether the lazy sums
+  // fold after it (every second group).
+  auto group = [&](const S (&e)[K][4], T w, bool fold) {
+    S A[K], B[K], dA[K], dB[K];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const size_t g = g0 + (size_t)u * stride;
-      if (g < q4) {
-        T w0a = FC::one(), w0b = FC::one(), w1 = FC::one();
-        if constexpr (PW) {
-          w0a = ca[u];
-          w0b = cb[u];
-          w1 = cc[u];
-        } else if constexpr (EQ) {
-          w0a = eq_w<FC>(wl0, wr0, x0, false, g);
-          w0b = eq_w<FC>(wl0, wr0, x0, false, g + q4);
-          w1 = eq_w<FC>(wl1, wr1, x1, false, g);
-        }
-        const bool fold = (par++ & 1u) != 0;
-        // Round 0: pairs (x0, x2) and (x1, x3).
+    for (int q = 0; q < K; ++q) {
+      A[q] = e[q][0];
+      B[q] = FT::sub(e[q][2], e[q][0]);
+      dA[q] = FT::sub(e[q][1], e[q][0]);
+      dB[q] = FT::sub(FT::sub(e[q][3], e[q][1]), B[q]);
+    }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          S lo[K], df[K];
+    for (int t = 0; t < NA; ++t) {
+      S p[K + 1];
+      la_poly<FT, K>(A, B, p);
 #pragma unroll
-          for (int q = 0; q < K; ++q) {
-            lo[q] = cur[u][q][h];
-            df[q] = FT::sub(cur[u][q][h + 2], cur[u][q][h]);
-          }
-#pragma unroll
-          for (int t = 0; t < NA; ++t) {
-            S prod = lo[0];
-#pragma unroll
-            for (int q = 1; q < K; ++q) prod = FT::mul(prod, lo[q]);
-            dsum[t].add(h ? w0b : w0a, prod, h == 1);
-            if (t < K) {
-#pragma unroll
-              for (int q = 0; q < K; ++q) lo[q] = FT::add(lo[q], df[q]);
-            }
-          }
-        }
-        // Round 1's pair g: lo = x0 + r (x2 - x0), hi = x1 + r (x3 - x1).
-        S A[K], B[K], dA[K], dB[K];
+      for (int i = 0; i < NA; ++i) csum[t][i].add(w, p[i], fold);
+      if (t < K) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-          A[q] = cur[u][q][0];
-          B[q] = FT::sub(cur[u][q][2], cur[u][q][0]);
-          dA[q] = FT::sub(cur[u][q][1], cur[u][q][0]);
-          dB[q] = FT::sub(FT::sub(cur[u][q][3], cur[u][q][1]), B[q]);
-        }
-#pragma unroll
-        for (int t = 0; t < NA; ++t) {
-          S p[K + 1];
-          la_poly<FT, K>(A, B, p);
-#pragma unroll
-          for (int i = 0; i < NA; ++i) csum[t][i].add(w1, p[i], fold);
-          if (t < K) {
-#pragma unroll
-            for (int q = 0; q < K; ++q) {
-              A[q] = FT::add(A[q], dA[q]);
-              B[q] = FT::add(B[q], dB[q]);
-            }
-          }
+          A[q] = FT::add(A[q], dA[q]);
+          B[q] = FT::add(B[q], dB[q]);
         }
       }
-      __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto weight = [&](uint32_t g) -> T {
+    if constexpr (EQ) return eq_w<FC>(wl1, wr1, x1, false, g);
+    return FC::one();
+  };
+  // Steps whose U groups are all in the round (start < stride).
+  const uint32_t nfull = q4 / step;
+  uint32_t g0 = start;
+  S cur[U][K][4];
+  T cc[UW];
+  if (nfull > 0) load(g0, cur, cc);
+  for (uint32_t k = 0; k < nfull; ++k, g0 += step) {
+    S nxt[U][K][4];
+    T nc[UW];
+    load(g0 + step, nxt, nc);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      group(cur[u], PW ? cc[u] : weight(g0 + (uint32_t)u * stride), (u & 1) || u == U - 1);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int q = 0; q < K; ++q)
 #pragma unroll
         for (int m = 0; m < 4; ++m) cur[u][q][m] = nxt[u][q][m];
-      if constexpr (PW) {
-        ca[u] = na[u];
-        cb[u] = nb[u];
-        cc[u] = nc[u];
+      if constexpr (PW) cc[u] = nc[u];
+    }
+  }
+  // The last, partial step.
+  if (g0 < q4) {
+    load(g0, cur, cc);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = g0 + (uint32_t)u * stride;
+      if (g < q4) group(cur[u], PW ? cc[u] : weight(g), true);
+    }
+  }
+}
+
+// The entries of one 16-byte vector (16 / sizeof(S) consecutive elements).
+template <class S>
+__device__ __forceinline__ void unpack16(const uint4& v, S (&o)[16 / sizeof(S)]) {
+  static_assert(16 % sizeof(S) == 0, "element size");
+  __builtin_memcpy(&o[0], &v, 16);
+}
+
+// fused01_kernel's streaming loop over 16-byte vectors: a lane loads VG =
+// 16 / sizeof(S) consecutive groups of every stream at once (a wave reads
+// 1 KiB a stream per load). Round 1's eq weight of group g is L[g >> x] R[g &
+// (2^x - 1)] (or R[g] alone, or none); the groups are split into H = 2^(n-2-x)
+// blocks of equal L, and P = T / H threads share a block, each taking chunks
+// cl0, cl0 + P, ... of it, so that L is a per-thread factor (applied to the
+// sums at the end, as `lfac`) and R the per-group weight, read with the
+// entries (R is 2^x elements: cache-resident).
+template <class FT, class FC, int K, bool EQ>
+__device__ __forceinline__ void fused01_stream_vec(const StageArgs<FT, FC>& a, uint32_t t,
+                                                   uint32_t P, uint32_t xr, const typename FC::T* R,
+                                                   FusedSum<FC, FT, EQ> (&csum)[K + 1][K + 1]) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1;
+  constexpr int VG = 16 / (int)sizeof(S);
+  constexpr uint32_t LVG = VG == 4 ? 2u : VG == 2 ? 1u : 0u;
+  constexpr int NW = EQ ? VG : 1;
+  const uint32_t q4 = 1u << (a.n - 2);
+  const uint32_t hb = 1u << (xr - LVG);  // chunks of one block of equal L
+  const uint32_t h = t / P, cl0 = t % P, ns = hb / P;
+  const char* base[K][4];
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      base[q][m] = reinterpret_cast<const char*>(a.in + (size_t)q * a.in_stride + (size_t)m * q4);
+  // Step s's chunk (steps past the last reload the last: in bounds).
+  auto load = [&](uint32_t s, uint4 (&x)[K][4], T (&w)[NW]) {
+    const uint32_t cl = cl0 + min(s, ns - 1u) * P;
+    const uint32_t off = ((h * hb + cl) << 4);
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) x[q][m] = *reinterpret_cast<const uint4*>(base[q][m] + off);
+    if constexpr (EQ) {
+      const T* r = R + (cl << LVG);
+#pragma unroll
+      for (int v = 0; v < VG; ++v) w[v] = r[v];
+    }
+  };
+  auto group = [&](const S (&e)[K][4], T w, bool fold) {
+    S A[K], B[K], dA[K], dB[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      A[q] = e[q][0];
+      B[q] = FT::sub(e[q][2], e[q][0]);
+      dA[q] = FT::sub(e[q][1], e[q][0]);
+      dB[q] = FT::sub(FT::sub(e[q][3], e[q][1]), B[q]);
+    }
+#pragma unroll
+    for (int tt = 0; tt < NA; ++tt) {
+      S p[K + 1];
+      la_poly<FT, K>(A, B, p);
+#pragma unroll
+      for (int i = 0; i < NA; ++i) csum[tt][i].add(w, p[i], fold);
+      if (tt < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          A[q] = FT::add(A[q], dA[q]);
+          B[q] = FT::add(B[q], dB[q]);
+        }
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  uint4 cur[K][4];
+  T cw[NW];
+  load(0, cur, cw);
+  for (uint32_t s = 0; s < ns; ++s) {
+    uint4 nxt[K][4];
+    T nw[NW];
+    load(s + 1, nxt, nw);
+    S el[K][4][VG];
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) unpack16<S>(cur[q][m], el[q][m]);
+#pragma unroll
+    for (int v = 0; v < VG; ++v) {
+      S e[K][4];
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) e[q][m] = el[q][m][v];
+      group(e, EQ ? cw[EQ ? v : 0] : FC::one(), (v & 1) || v == VG - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) cur[q][m] = nxt[q][m];
+#pragma unroll
+    for (int v = 0; v < NW; ++v) cw[v] = nw[v];
   }
 }
 
@@ -2480,16 +2577,34 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
     if (!sh.is_last) return;
     StageArgs<FT, FC> af = a;
     af.final_out = nullptr;
-    slot_rows_sum<FC, NA>(a.slots, sh);
-    if (tid == 0) trace_mark(a.trace, 0, 2);
-    if (tid < 64)
-      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 0, false, sh.out, sh.sm, a.work, a.in,
-                                             a.in_stride, true);
-    // Round 1's node sums: its coefficients at r_0.
     static_assert(sizeof(sh.sum64) == (PASS_THREADS / LA_SLOTW) * LA_SLOTW * sizeof(uint64_t),
                   "slot scratch shapes");
     la_slot_sum<FC, NC>(la_rows(a.slots, 1), NA * W,
                         reinterpret_cast<uint64_t (*)[LA_SLOTW]>(&sh.sum64[0][0]), sh.out);
+    // Round 0's node sums from round 1's coefficients (fused01_stream):
+    // node t = sum_i t^i (e0 c[0][i] + e1 c[1][i]), e_b = eq(w_1, b).
+    if (tid < (uint32_t)NA) {
+      T e0 = FC::one(), e1 = FC::one();
+      if constexpr (EQ) {
+        e0 = a.eq_ab[3];                   // 1 - w_1
+        e1 = FC::add(a.eq_ab[2], a.eq_ab[3]);  // w_1
+      }
+      T v = FC::zero();
+#pragma unroll
+      for (int i = K; i >= 0; --i) {
+        const T u = FC::add(FC::mul(e0, sh.out[i]), FC::mul(e1, sh.out[NA + i]));
+        T vt = FC::zero();
+        for (uint32_t j = 0; j < tid; ++j) vt = FC::add(vt, v);  // v * t, t <= K
+        v = FC::add(vt, u);
+      }
+      sh.lds[tid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) trace_mark(a.trace, 0, 2);
+    if (tid < 64)
+      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 0, false, sh.lds, sh.sm, a.work, a.in,
+                                             a.in_stride, true);
+    // Round 1's node sums: its coefficients at r_0.
     if (tid < (uint32_t)NA) {
       const T r = sh.sm.r;
       T v = sh.out[tid * NA + K];
@@ -2511,45 +2626,49 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
     return;
   }
   const uint32_t blk = blockIdx.x - 1;
-  const size_t q4 = (size_t)1 << (n - 2);  // round 1's pairs
-  const size_t start = (size_t)blk * PASS_THREADS + tid, stride = (size_t)nblk * PASS_THREADS;
-  const T* wl0 = nullptr;
-  const T* wr0 = nullptr;
+  const uint32_t q4 = 1u << (n - 2);  // round 1's pairs
+  const uint32_t start = blk * PASS_THREADS + tid, stride = nblk * PASS_THREADS;
   const T* wl1 = nullptr;
   const T* wr1 = nullptr;
-  uint32_t x0 = 0, x1 = 0;
+  uint32_t x1 = 0;
   bool fac = false;
   if constexpr (EQ) {
-    select_eq<FC>(0, n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl0, wr0, x0);
     select_eq<FC>(1, n, a.eq_l_fam, a.eq_r_fam, &ctl->one, wl1, wr1, x1);
-    const size_t mask = ((size_t)1 << x0) - 1;
-    fac = wl0 && wl1 && wr0 == wr1 && x0 == x1 && (stride & mask) == 0 && (q4 & mask) == 0;
+    const uint32_t mask = (1u << x1) - 1u;
+    fac = wl1 && (stride & mask) == 0;
   }
-  FusedSum<FC, FT, EQ> dsum[NA], csum[NA][NA];
+  FusedSum<FC, FT, EQ> csum[NA][NA];
 #pragma unroll
-  for (int t = 0; t < NA; ++t) {
-    dsum[t].init();
+  for (int t = 0; t < NA; ++t)
 #pragma unroll
     for (int i = 0; i < NA; ++i) csum[t][i].init();
+  // The 16-byte-vector loop when the groups split into blocks of equal L
+  // (one per P threads, P a power of two no larger than a block's chunks).
+  constexpr uint32_t LVG = sizeof(S) == 4 ? 2u : sizeof(S) == 8 ? 1u : 0u;
+  const uint32_t xr = EQ && wl1 ? x1 : n - 2;
+  const uint32_t nch = q4 >> LVG;
+  const uint32_t H = xr >= LVG ? nch >> (xr - LVG) : 0u;
+  const bool vec = H != 0 && H <= stride && (stride / H) <= (1u << (xr - LVG));
+  T fac_w = FC::one();
+  if (vec) {
+    const uint32_t P = stride / H;
+    fused01_stream_vec<FT, FC, K, EQ>(a, start, P, xr, wr1, csum);
+    if (EQ && wl1) fac_w = wl1[start / P];
+  } else if (fac) {
+    fused01_stream<FT, FC, K, EQ, true>(a, start, stride, wl1, wr1, x1, csum);
+    fac_w = start < q4 ? wr1[start & ((1u << x1) - 1u)] : FC::one();
+  } else {
+    fused01_stream<FT, FC, K, EQ, false>(a, start, stride, wl1, wr1, x1, csum);
   }
-  if (fac)
-    fused01_stream<FT, FC, K, EQ, true>(a, start, stride, wl0, wr0, x0, wl1, wr1, x1, dsum, csum);
-  else
-    fused01_stream<FT, FC, K, EQ, false>(a, start, stride, wl0, wr0, x0, wl1, wr1, x1, dsum, csum);
-  T d[NA], c[NC];
-  const T wrc = EQ && fac && start < q4 ? wr0[start & (((size_t)1 << x0) - 1)] : FC::one();
+  T c[NC];
 #pragma unroll
   for (int t = 0; t < NA; ++t) {
-    d[t] = dsum[t].get();
-    if (EQ && fac) d[t] = FC::mul(d[t], wrc);
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       c[t * NA + i] = csum[t][i].get();
-      if (EQ && fac) c[t * NA + i] = FC::mul(c[t * NA + i], wrc);
+      if (EQ) c[t * NA + i] = FC::mul(c[t * NA + i], fac_w);
     }
   }
-  block_reduce<FC, NA, PASS_THREADS>(d, sh.lds, sh.out);
-  slot_row_add<FC, NA>(a.slots, blk, sh.out);
   block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
   la_slot_add<FC, NC>(la_rows(a.slots, 1), blk, NA * W, sh.out);
   __builtin_amdgcn_s_waitcnt(0);
