@@ -809,6 +809,36 @@ __device__ __forceinline__ void fin_store(const FinishLds<FC>& sm, Ctl<FC>* ctl)
   }
 }
 
+// The claim a round reduces to at r from its message `msg` (LDS, the wire
+// form's elements) and the claim before it; every lane of the wave gets it.
+// Values on the wire: s(r) = sum_t s(t) L_t(r), L_t(r) = lag_inv[t]
+// prod_{j != t} (r - j), lane t (of each eight-lane group) forming term t.
+// Another wire form: its reduction map, lane k forming c_k r^k.
+template <class FC, int NPTS>
+__device__ __forceinline__ typename FC::T round_claim(const FinishLds<FC>& sm,
+                                                      const typename FC::T* msg,
+                                                      typename FC::T r, typename FC::T claim) {
+  using T = typename FC::T;
+  const uint32_t t = threadIdx.x & 7u;
+  const uint32_t tn = t < (uint32_t)NPTS ? t : (uint32_t)(NPTS - 1);
+  if (sm.wire == WIRE_VALUES) {
+    T basis = sm.lag_inv[tn];
+#pragma unroll
+    for (int j = 0; j < NPTS; ++j) {
+      const T d = FC::sub(r, sm.nodes[j]);
+      basis = FC::mul(basis, t == (uint32_t)j ? FC::one() : d);
+    }
+    const T m = t < (uint32_t)NPTS ? msg[tn] : FC::zero();
+    return lanes8_sum<FC>(FC::mul(m, basis));
+  }
+  const uint32_t kk = t < sm.ncoef ? t : 0u;
+  T c = FC::mul(sm.cclaim[kk], claim);
+  for (uint32_t j = 0; j < sm.mlen; ++j) c = FC::add(c, FC::mul(sm.cmat[kk][j], msg[j]));
+  T rk = FC::one();
+  for (uint32_t i = 0; i < kk; ++i) rk = FC::mul(rk, r);
+  return lanes8_sum<FC>(t < sm.ncoef ? FC::mul(c, rk) : FC::zero());
+}
+
 // Wave 0 only. Lane t = lane & 7 works on node t (the eight-lane groups of
 // the wave agree): message from the node sums, Fiat-Shamir hop, bind, eq-mass
 // update, the claim reduction when `need_claim`, and (last round) the final
@@ -882,28 +912,8 @@ __device__ __forceinline__ void finish_round(const StageArgs<FS, FC>& a, uint32_
   T eq_next = FC::zero();
   if constexpr (EQ) eq_next = FC::mul(sm.eq_prev, FC::add(sm.eqB, FC::mul(r, sm.eqA)));
   T claim = FC::zero();
-  const bool wire_values = sm.wire == WIRE_VALUES;
   need_claim = need_claim || sm.claim_every != 0;
-  if (need_claim && wire_values) {
-    // s(r) = sum_t s(t) L_t(r) with L_t(r) = lag_inv[t] prod_{j != t} (r - j):
-    // lane t forms term t; lanes past the nodes add zero.
-    T basis = sm.lag_inv[tn];
-#pragma unroll
-    for (int j = 0; j < NPTS; ++j) {
-      const T d = FC::sub(r, sm.nodes[j]);
-      basis = FC::mul(basis, t == (uint32_t)j ? FC::one() : d);
-    }
-    const T msg = t < (uint32_t)NPTS ? sm.msg[tn] : FC::zero();
-    claim = lanes8_sum<FC>(FC::mul(msg, basis));
-  } else if (need_claim) {
-    // The wire form's reduction: lane k forms c_k r^k.
-    const uint32_t kk = t < sm.ncoef ? t : 0u;
-    T c = FC::mul(sm.cclaim[kk], sm.claim);
-    for (uint32_t j = 0; j < sm.mlen; ++j) c = FC::add(c, FC::mul(sm.cmat[kk][j], sm.msg[j]));
-    T rk = FC::one();
-    for (uint32_t i = 0; i < kk; ++i) rk = FC::mul(rk, r);
-    claim = lanes8_sum<FC>(t < sm.ncoef ? FC::mul(c, rk) : FC::zero());
-  }
+  if (need_claim) claim = round_claim<FC, NPTS>(sm, sm.msg, r, sm.claim);
   wave_sync();
   if (lane == 0) {
     sm.r = r;
@@ -1559,7 +1569,7 @@ __device__ __forceinline__ uint64_t* la_rows(uint64_t* slots, uint32_t x) {
 // limit marks the proof faulted.
 constexpr int LA_THREADS = 512;
 constexpr uint32_t LA_FIRST_WAVES = LA_THREADS / 64;  // waves of the first period
-constexpr uint32_t LA_WAVES = 6;                      // waves of the later periods
+constexpr uint32_t LA_WAVES = 5;                      // waves of the later periods
 constexpr uint32_t LA_SPIN_LIMIT = 1u << 20;
 
 template <class FC, int K>
@@ -1575,7 +1585,72 @@ struct TailLaLds {
   uint32_t go;           // periods allowed to start: every k < go
   uint32_t arrived;      // wave arrivals over all periods so far
   uint32_t fault;
+  // The helper wave's side (rounds after the first): round k's message as
+  // polynomials in r_{k-1} (by parity), the coefficients it sums them from,
+  // the messages and challenges by parity for its claim and eq bookkeeping.
+  typename FC::T mp[2][MAXPTS][MAXPTS + 1];
+  typename FC::T pnat[MAXPTS][MAXPTS + 1];
+  typename FC::T hcoef[NA * NA];
+  typename FC::T msgv[2][MAXPTS];
+  typename FC::T rpub[2];
+  uint32_t mp_ready;  // rounds whose message polynomials are in mp: every k <= mp_ready
+  uint32_t booked;    // rounds whose eq and claim bookkeeping is done: every k < booked
 };
+
+// Round k's message as polynomials in x = r_{k-1} (the tail's helper wave,
+// all lanes): node t's sum is V_t(x) = sum_i C[t][i] x^i (C the coefficients
+// of round k's node sums, t <= K; the extension node's V is the extension of
+// those), and with eq the message value is l_{k-1}(x) E L_t V_t(x), where E
+// is the eq product before round k-1, l_{k-1}(x) = B0 + x A0 its line and
+// L_t = B1 + node_t A1 round k's line at node t. mp[w][j] is the coefficient
+// of x^j of wire element w (row w of the wire map applied to the nodes').
+template <class FC, int K, bool EQ>
+__device__ __forceinline__ void tail_msg_poly(const FinishLds<FC>& sm, const typename FC::T* C,
+                                              typename FC::T E, typename FC::T A0,
+                                              typename FC::T B0, typename FC::T A1,
+                                              typename FC::T B1,
+                                              typename FC::T (*mp)[MAXPTS + 1],
+                                              typename FC::T (*pn)[MAXPTS + 1]) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  constexpr int D = EQ ? K + 1 : K;  // degree in x
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool values = sm.wire == WIRE_VALUES;
+  T (*dst)[MAXPTS + 1] = values ? mp : pn;
+  auto vt = [&](uint32_t t, int i) -> T {
+    if (i < 0 || i > K) return FC::zero();
+    if (t <= (uint32_t)K) return C[t * NA + i];
+    T v = FC::zero();
+#pragma unroll
+    for (int u = 0; u < NA; ++u) v = FC::add(v, FC::mul(sm.ext_coef[u], C[u * NA + i]));
+    return v;
+  };
+  for (uint32_t idx = lane; idx < (uint32_t)(NPTS * (D + 1)); idx += 64u) {
+    const uint32_t t = idx / (D + 1);
+    const int j = (int)(idx % (D + 1));
+    T v;
+    if constexpr (EQ) {
+      const T Lt = FC::add(B1, FC::mul(sm.nodes[t], A1));
+      const T e0 = FC::mul(E, B0), e1 = FC::mul(E, A0);
+      v = FC::mul(Lt, FC::add(FC::mul(e0, vt(t, j)), FC::mul(e1, vt(t, j - 1))));
+    } else {
+      v = vt(t, j);
+    }
+    dst[t][j] = v;
+  }
+  wave_sync();
+  if (!values) {
+    for (uint32_t idx = lane; idx < sm.mlen * (uint32_t)(D + 1); idx += 64u) {
+      const uint32_t w = idx / (D + 1), j = idx % (D + 1);
+      T v = FC::zero();
+#pragma unroll
+      for (int t = 0; t < NPTS; ++t) v = FC::add(v, FC::mul(sm.wmat[w][t], pn[t][j]));
+      mp[w][j] = v;
+    }
+    wave_sync();
+  }
+}
 
 // Spin on an LDS word until it reaches `target` (relaxed polls, then an
 // acquire fence for the work-group). False when the limit is reached.
@@ -1585,6 +1660,19 @@ __device__ __forceinline__ bool la_wait(uint32_t* p, uint32_t target) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       return true;
     }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return false;
+}
+
+// la_wait that also gives up (false) once `fault` is set.
+__device__ __forceinline__ bool la_wait_f(uint32_t* p, uint32_t target, uint32_t* fault) {
+  for (uint32_t i = 0; i < LA_SPIN_LIMIT; ++i) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      return true;
+    }
+    if (__hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
   return false;
@@ -1792,10 +1880,17 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
     sh.arrived = 0;
     sh.fault = 0;
     sh.pub_r = a.ctl->r;
+    sh.mp_ready = first;
+    sh.booked = 0;
   }
   __syncthreads();
-  // Wave index among the later periods' waves (1-3, 5-7 -> 0..5), or -1.
-  const int widx = wave == 0 || wave == 4 ? -1 : (int)(wave < 4 ? wave - 1 : wave - 2);
+  // Wave index among the later periods' waves (1-3, 5, 6 -> 0..4), or -1.
+  // Waves w and w + 4 share a SIMD: the finisher (wave 0) has its SIMD to
+  // itself, the periods take waves 1-3, 5, 6 and the helper wave 7.
+  constexpr uint32_t HELPER_WAVE = 7;
+  const int widx = wave == 0 || wave == 4 || wave == HELPER_WAVE
+                       ? -1
+                       : (int)(wave < 4 ? wave - 1 : wave - 2);
   const uint32_t q8f = 1u << (n - first - 2);  // pairs of round first+1
   // Every wave: the first period's fold and round first's node sums.
   if (!cont) {
@@ -1848,9 +1943,63 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
     }
     return;
   }
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  constexpr int D = EQ ? K + 1 : K;  // degree of a message polynomial
+  const uint32_t base_t = cont ? 0u : LA_FIRST_WAVES;
+  if (wave == HELPER_WAVE) {
+    // The helper: while the finisher hashes round k-1, form round k's
+    // message as polynomials in r_{k-1} from period k-1's coefficients, so
+    // that the finisher only evaluates them at r_{k-1}; once r_{k-1} is out,
+    // the eq product and (when needed) the claim of round k-1.
+    if (first + 1 >= n) return;
+    T E = EQ ? a.ctl->eq_prev : FC::one();  // eq product before round k-1
+    T claim = FC::zero();
+    for (uint32_t k = first + 1; k <= n; ++k) {
+      if (k < n) {
+        if (!la_wait_f(&sh.arrived, base_t + (k - first) * LA_WAVES, &sh.fault)) break;
+        if (lane < (uint32_t)NC) {
+          T v = FC::zero();
+          for (uint32_t w = 0; w < LA_WAVES; ++w) v = FC::add(v, sh.part[(k - 1) & 1][w][NA + lane]);
+          sh.hcoef[lane] = v;
+        }
+        wave_sync();
+        T A0 = FC::zero(), B0 = FC::zero(), A1 = FC::zero(), B1 = FC::zero();
+        if constexpr (EQ) {
+          A0 = sh.sm.eqab[2 * (k - 1)];
+          B0 = sh.sm.eqab[2 * (k - 1) + 1];
+          A1 = sh.sm.eqab[2 * k];
+          B1 = sh.sm.eqab[2 * k + 1];
+        }
+        tail_msg_poly<FC, K, EQ>(sh.sm, sh.hcoef, E, A0, B0, A1, B1, sh.mp[k & 1], sh.pnat);
+        if (lane == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __hip_atomic_store(&sh.mp_ready, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+      if (!la_wait_f(&sh.go, k + 1, &sh.fault)) break;
+      const T r = sh.rpub[(k - 1) & 1];
+      if (k - 1 == first) {
+        claim = sh.sm.claim;  // the finisher's own reduction of round `first`
+      } else if (k == n || sh.sm.claim_every) {
+        claim = round_claim<FC, NPTS>(sh.sm, sh.msgv[(k - 1) & 1], r, claim);
+      }
+      if constexpr (EQ)
+        E = FC::mul(E, FC::add(sh.sm.eqab[2 * (k - 1) + 1], FC::mul(r, sh.sm.eqab[2 * (k - 1)])));
+      if (k == n && lane == 0) {
+        sh.sm.eq_prev = E;
+        sh.sm.claim = claim;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&sh.booked, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      wave_sync();
+    }
+    if (__hip_atomic_load(&sh.booked, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < n && lane == 0)
+      __hip_atomic_store(&sh.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return;
+  }
   if (wave != 0) return;
   // The finisher.
-  uint32_t target = cont ? 0u : LA_FIRST_WAVES;
+  uint32_t target = base_t;
   bool ok = true;
   for (uint32_t k = first; k < n && ok; ++k) {
     if (lane == 0) trace_mark(a.trace, k, 1);
@@ -1893,40 +2042,55 @@ __global__ __launch_bounds__(LA_THREADS) void tail_la_kernel(StageArgs<FS, FC> a
         sh.out[lane] = s;
       }
     } else {
-      // Round k's node sums: the coefficients of period k-1 at r_{k-1}.
-      const uint32_t par = (k - 1) & 1u;
-      if (lane < (uint32_t)NC) {
-        T s = FC::zero();
-        for (uint32_t w = 0; w < LA_WAVES; ++w) s = FC::add(s, sh.part[par][w][NA + lane]);
-        sh.coef[lane] = s;
+      // Round k's message: the helper's polynomials at r_{k-1}; the hop;
+      // the challenge (the helper keeps the eq product and the claim).
+      ok = la_wait_f(&sh.mp_ready, k, &sh.fault);
+      if (!ok) break;
+      const uint32_t par = k & 1u;
+      const uint32_t mlen = sh.sm.mlen;
+      if (lane < mlen) {
+        const T r = sh.sm.r;
+        T v = sh.mp[par][lane][D];
+#pragma unroll
+        for (int j = D - 1; j >= 0; --j) v = FC::add(FC::mul(v, r), sh.mp[par][lane][j]);
+        a.msgs[(size_t)k * NPTS + lane] = v;
+        sh.sm.msg[lane] = v;
+        sh.msgv[par][lane] = v;
       }
       wave_sync();
-      if (lane < (uint32_t)NA) {
-        const T r = sh.sm.r;
-        T v = sh.coef[lane * NA + K];
-#pragma unroll
-        for (int i = K - 1; i >= 0; --i) v = FC::add(sh.coef[lane * NA + i], FC::mul(r, v));
-        sh.out[lane] = v;
+      if (lane == 0) trace_mark(a.trace, k, 4);
+      fs_hop_body(sh.sm, reinterpret_cast<const uint8_t*>(sh.sm.msg), mlen * (uint32_t)sizeof(T),
+                  a.ctl->item_bytes, a.ctl->limbs, a.ctl->tr_kind);
+      if (lane == 0) trace_mark(a.trace, k, 5);
+      if (lane < (uint32_t)sizeof(T)) a.point[(size_t)k * sizeof(T) + lane] = sh.sm.out[lane];
+      const T r = el_from_words<FC>(reinterpret_cast<const uint32_t*>(sh.sm.out));
+      if (lane == 0) {
+        sh.sm.r = r;
+        trace_mark(a.trace, k, 6);
       }
+      wave_sync();
     }
-    wave_sync();
-    StageArgs<FS, FC> af = a;
-    af.final_out = nullptr;  // the final fold waits for the last period
-    finish_round<FS, FC, K, EQ, HOP_INLINE>(af, k, k + 1 == n, sh.out, sh.sm, a.work, a.in,
-                                            a.work_stride, false);
-    // Period k is complete: publish r_k and start period k+1.
+    if (k == first) {
+      wave_sync();
+      StageArgs<FS, FC> af = a;
+      af.final_out = nullptr;  // the final fold waits for the last period
+      finish_round<FS, FC, K, EQ, HOP_INLINE>(af, k, k + 1 == n, sh.out, sh.sm, a.work, a.in,
+                                              a.work_stride, false);
+    }
+    // Period k is complete: publish r_k (for the helper too) and start
+    // period k+1.
     target += LA_WAVES;
     ok = la_wait(&sh.arrived, target);
     if (!ok) break;
-    if (k + 1 < n) {
-      if (lane == 0) {
-        sh.pub_r = sh.sm.r;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __hip_atomic_store(&sh.go, k + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      wave_sync();
+    if (lane == 0) {
+      sh.pub_r = sh.sm.r;
+      sh.rpub[k & 1u] = sh.sm.r;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __hip_atomic_store(&sh.go, k + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    wave_sync();
   }
+  if (ok && first + 1 < n) ok = la_wait_f(&sh.booked, n, &sh.fault);
   if (!ok) {
     if (lane == 0) {
       __hip_atomic_store(&sh.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -2025,6 +2189,34 @@ __device__ __forceinline__ void la_slot_sum(uint64_t* rows, uint32_t word_off,
   __syncthreads();
 }
 
+// la_slot_sum for one wave (all lanes): words [word_off, word_off + NV W)
+// of the rows into out[0..NV), clearing them; `scratch` (LDS) holds NV W
+// words.
+template <class FC, int NV>
+__device__ __forceinline__ void la_slot_sum_wave(uint64_t* rows, uint32_t word_off,
+                                                 uint64_t* scratch, typename FC::T* out) {
+  constexpr int W = SlotOps<FC>::W;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t w = lane; w < (uint32_t)(NV * W); w += 64u) {
+    uint64_t v[LA_NSLOT];
+#pragma unroll
+    for (int i = 0; i < LA_NSLOT; ++i)
+      v[i] = __hip_atomic_load(rows + (size_t)i * LA_SLOTW + word_off + w, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < LA_NSLOT; ++i) {
+      t += v[i];
+      __hip_atomic_store(rows + (size_t)i * LA_SLOTW + word_off + w, (uint64_t)0,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    scratch[w] = t;
+  }
+  wave_sync();
+  if (lane < (uint32_t)NV) out[lane] = SlotOps<FC>::join(&scratch[lane * W]);
+  wave_sync();
+}
+
 template <class FC, int K>
 struct ChainLaLds {
   static constexpr int NA = K + 1, NC = NA * NA;
@@ -2034,6 +2226,20 @@ struct ChainLaLds {
   uint64_t sum64[PASS_THREADS / LA_SLOTW][LA_SLOTW];
   typename FC::T r;
   uint32_t ok;
+  // Block 0: the finisher's slot words, and the helper wave's side as in the
+  // look-ahead tail.
+  uint64_t fsum[NC * 4];
+  uint64_t hsum[NC * 4];
+  typename FC::T mp[2][MAXPTS][MAXPTS + 1];
+  typename FC::T pnat[MAXPTS][MAXPTS + 1];
+  typename FC::T hcoef[NC];
+  typename FC::T msgv[2][MAXPTS];
+  typename FC::T rpub[2];
+  uint32_t mp_ready;  // rounds whose message polynomials are in mp: every k <= mp_ready
+  uint32_t pdone;     // periods complete: every k < pdone
+  uint32_t go_h;      // rounds whose challenge is in rpub: every k < go_h
+  uint32_t booked;    // rounds whose eq and claim bookkeeping is done: every k < booked
+  uint32_t hfault;
 };
 
 template <class FT, class FC, int K, bool EQ>
@@ -2050,66 +2256,173 @@ __global__ __launch_bounds__(PASS_THREADS) void chain_la_kernel(
   // sums coefficients; else its fold-and-sums part, then its coefficients.
   const uint32_t nb0 = la_blocks(cont ? la_units(n, first) : 1u << (n - first - 2), G);
   if (blockIdx.x == 0) {
-    // The finisher.
+    // Block 0: the finisher (wave 0) and its helper (wave 1, another SIMD).
+    // The helper waits for each period's arrivals, sums the coefficients it
+    // left, forms the next round's message as polynomials in the pending
+    // challenge (tail_msg_poly) and keeps the eq product and the claim, so
+    // that after each hop the finisher only publishes the challenge and
+    // evaluates the next message.
+    constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+    constexpr int D = EQ ? K + 1 : K;
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
     if (tid < 64) {
       fin_load(sh.sm, ctl);
       fin_eq_table<FT, FC, EQ>(sh.sm, a);
     }
-    if (tid == 0) trace_mark(a.trace, first, 7);
-    uint32_t target = cont ? 0u : nb0;
-    for (uint32_t k = first; k < last; ++k) {
-      if (tid < 64) fin_eq_lds<FC, EQ>(sh.sm, k);
-      if (k == first && !cont) {
-        if (tid == 0) sh.ok = chain_wait(&ctl->counter, target, &ctl->fault) ? 1u : 0u;
-        __syncthreads();
-        if (!sh.ok) return;
-        la_slot_sum<FC, NA>(la_rows(a.slots, first), 0, sh.sum64, sh.out);
-      } else {
-        // Round k's node sums: its coefficients (from the previous period,
-        // or with `cont` the previous launch) at r_{k-1}.
-        la_slot_sum<FC, NC>(la_rows(a.slots, k), NA * W, sh.sum64, sh.out);
-        if (tid < (uint32_t)NA) {
-          const T r = sh.sm.r;
-          T v = sh.out[tid * NA + K];
-#pragma unroll
-          for (int i = K - 1; i >= 0; --i) v = FC::add(sh.out[tid * NA + i], FC::mul(r, v));
-          sh.lds[tid] = v;
+    if (tid == 0) {
+      trace_mark(a.trace, first, 7);
+      sh.mp_ready = first;
+      sh.pdone = first;
+      sh.go_h = first;
+      sh.booked = 0;
+      sh.hfault = 0;
+    }
+    __syncthreads();
+    if (wave == 1) {
+      T E = EQ ? ctl->eq_prev : FC::one();  // eq product before round k-1
+      T claim = FC::zero();
+      uint32_t target = cont ? 0u : nb0;
+      bool ok = true;
+      for (uint32_t k = first + 1; k <= last && ok; ++k) {
+        // Period k-1 complete; round k's message polynomials from the
+        // coefficients it left.
+        target += k - 1 == first ? nb0 : la_blocks(la_units(n, k - 1), G);
+        ok = chain_wait(&ctl->counter, target, &ctl->fault);
+        if (!ok) break;
+        // At once: the finisher publishes r_{k-1} (the start of period k) on it.
+        if (lane == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __hip_atomic_store(&sh.pdone, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        __syncthreads();
-        if (tid < (uint32_t)NA) sh.out[tid] = sh.lds[tid];
-        __syncthreads();
+        if (k < last) {
+          la_slot_sum_wave<FC, NC>(la_rows(a.slots, k), NA * W, sh.hsum, sh.hcoef);
+          T A0 = FC::zero(), B0 = FC::zero(), A1 = FC::zero(), B1 = FC::zero();
+          if constexpr (EQ) {
+            A0 = sh.sm.eqab[2 * (k - 1)];
+            B0 = sh.sm.eqab[2 * (k - 1) + 1];
+            A1 = sh.sm.eqab[2 * k];
+            B1 = sh.sm.eqab[2 * k + 1];
+          }
+          tail_msg_poly<FC, K, EQ>(sh.sm, sh.hcoef, E, A0, B0, A1, B1, sh.mp[k & 1], sh.pnat);
+        }
+        if (k < last && lane == 0) {
+          // The cleared rows are visible before the additions of period k+1
+          // (which follow the finisher's publication of r_k).
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          __hip_atomic_store(&sh.mp_ready, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        // The eq product and the claim of round k-1, once r_{k-1} is out.
+        ok = la_wait_f(&sh.go_h, k, &sh.hfault);
+        if (!ok) break;
+        const T r = sh.rpub[(k - 1) & 1];
+        if (k - 1 == first) {
+          claim = sh.sm.claim;  // the finisher's own reduction of round `first`
+        } else if (k == claim_last || sh.sm.claim_every) {
+          claim = round_claim<FC, NPTS>(sh.sm, sh.msgv[(k - 1) & 1], r, claim);
+        }
+        if constexpr (EQ)
+          E = FC::mul(E, FC::add(sh.sm.eqab[2 * (k - 1) + 1], FC::mul(r, sh.sm.eqab[2 * (k - 1)])));
+        wave_sync();
       }
-      if (tid == 0) trace_mark(a.trace, k, 2);
-      if (tid < 64) {
+      if (lane == 0) {
+        if (ok) {
+          sh.sm.eq_prev = E;
+          sh.sm.claim = claim;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __hip_atomic_store(&sh.booked, last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          __hip_atomic_store(&sh.hfault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+      return;
+    }
+    if (wave != 0) return;
+    bool ok = true;
+    for (uint32_t k = first; k < last; ++k) {
+      fin_eq_lds<FC, EQ>(sh.sm, k);
+      if (k == first) {
+        if (!cont) {
+          ok = chain_wait(&ctl->counter, nb0, &ctl->fault);
+          if (!ok) break;
+          la_slot_sum_wave<FC, NA>(la_rows(a.slots, first), 0, sh.fsum, sh.out);
+        } else {
+          // Round first's node sums: the previous launch's coefficients at
+          // r_{first-1}.
+          la_slot_sum_wave<FC, NC>(la_rows(a.slots, first), NA * W, sh.fsum, sh.lds);
+          if (lane < (uint32_t)NA) {
+            const T r = sh.sm.r;
+            T v = sh.lds[lane * NA + K];
+#pragma unroll
+            for (int i = K - 1; i >= 0; --i) v = FC::add(sh.lds[lane * NA + i], FC::mul(r, v));
+            sh.out[lane] = v;
+          }
+          wave_sync();
+        }
+        if (lane == 0) trace_mark(a.trace, k, 2);
         StageArgs<FT, FC> af = a;
         af.final_out = nullptr;  // the final fold waits for the last period
-        finish_round<FT, FC, K, EQ, HOP_CHAIN>(af, k, k + 1 == claim_last, sh.out, sh.sm,
-                                               a.work, a.in, a.work_stride, false);
-      }
-      // Period k complete: publish r_k, which starts period k+1.
-      target += k == first ? nb0 : la_blocks(la_units(n, k), G);
-      if (tid == 0) {
-        bool ok = chain_wait(&ctl->counter, target, &ctl->fault);
-        if (ok && k + 1 < last) {
-          ctl->r = sh.sm.r;
-          __hip_atomic_store(&ctl->done, k - first + 1, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
+        finish_round<FT, FC, K, EQ, HOP_CHAIN>(af, k, k + 1 == claim_last, sh.out, sh.sm, a.work,
+                                               a.in, a.work_stride, false);
+      } else {
+        // Round k's message: the helper's polynomials at r_{k-1}; the hop.
+        ok = la_wait_f(&sh.mp_ready, k, &sh.hfault);
+        if (!ok) break;
+        const uint32_t par = k & 1u;
+        const uint32_t mlen = sh.sm.mlen;
+        if (lane < mlen) {
+          const T r = sh.sm.r;
+          T v = sh.mp[par][lane][D];
+#pragma unroll
+          for (int j = D - 1; j >= 0; --j) v = FC::add(FC::mul(v, r), sh.mp[par][lane][j]);
+          a.msgs[(size_t)k * NPTS + lane] = v;
+          sh.sm.msg[lane] = v;
+          sh.msgv[par][lane] = v;
         }
-        sh.ok = ok ? 1u : 0u;
+        wave_sync();
+        if (lane == 0) trace_mark(a.trace, k, 4);
+        fs_hop<HOP_CHAIN>((ZS_LDS HopLds*)static_cast<HopLds*>(&sh.sm),
+                          (const ZS_LDS uint8_t*)sh.sm.msg, mlen * (uint32_t)sizeof(T),
+                          ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+        if (lane == 0) trace_mark(a.trace, k, 5);
+        if (lane < (uint32_t)sizeof(T)) a.point[(size_t)k * sizeof(T) + lane] = sh.sm.out[lane];
+        const T r = el_from_words<FC>(reinterpret_cast<const uint32_t*>(sh.sm.out));
+        if (lane == 0) {
+          sh.sm.r = r;
+          trace_mark(a.trace, k, 6);
+        }
+        wave_sync();
       }
-      __syncthreads();
-      if (!sh.ok) return;
-    }
-    if (tid < 64) {
-      if (last == n && a.final_out && tid < (uint32_t)K) {
-        const T* s = a.work + (size_t)tid * a.work_stride;
-        const T lo = s[0], hi = s[1];
-        a.final_out[tid] = FC::add(lo, FC::mul(sh.sm.r, FC::sub(hi, lo)));
+      // Period k complete (the helper saw its arrivals): publish r_k, which
+      // starts period k+1.
+      ok = la_wait_f(&sh.pdone, k + 1, &sh.hfault);
+      if (!ok) break;
+      if (lane == 0) {
+        if (k + 1 < last) {
+          ctl->r = sh.sm.r;
+          __hip_atomic_store(&ctl->done, k - first + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        sh.rpub[k & 1u] = sh.sm.r;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&sh.go_h, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      if (EQ && last == n && a.final_out && tid == 0) a.final_out[K] = sh.sm.eq_prev;
-      fin_store(sh.sm, ctl);
+      wave_sync();
     }
-    if (tid == 0) {
+    if (ok) ok = la_wait_f(&sh.booked, last, &sh.hfault);
+    if (!ok) {
+      if (lane == 0) {
+        __hip_atomic_store(&sh.hfault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&ctl->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+    if (last == n && a.final_out && lane < (uint32_t)K) {
+      const T* s = a.work + (size_t)lane * a.work_stride;
+      const T lo = s[0], hi = s[1];
+      a.final_out[lane] = FC::add(lo, FC::mul(sh.sm.r, FC::sub(hi, lo)));
+    }
+    if (EQ && last == n && a.final_out && lane == 0) a.final_out[K] = sh.sm.eq_prev;
+    fin_store(sh.sm, ctl);
+    if (lane == 0) {
       __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
