@@ -47,6 +47,9 @@ extern "C" {
 /* Prover modes. */
 #define ZS_MODE_PRODUCT 1 /* StandardRound(ProductSummand(degree)), natural domain */
 #define ZS_MODE_EQ 2      /* EqPolyProver(ProductSummand(degree)), natural domain */
+#define ZS_MODE_LOGUP 3   /* LogupSumcheckRound (zorch/logup_gkr/prover.py:232-300): five
+                             stacked tables [eq, n0, d1, n1, d0] in one field, the summand
+                             eq (lam (n0 d1 + n1 d0) + d0 d1), degree 3, natural domain */
 
 typedef struct zs_ctx zs_ctx;
 typedef struct zs_state zs_state;
@@ -136,6 +139,8 @@ typedef struct zs_prove_args {
   const zs_wire* wire;       /* message wire form; null: ZS_WIRE_VALUES (copied) */
   const void* eq_mass;       /* ZS_MODE_EQ: the eq mass already bound, one challenge-field
                                 element (EqPolyState's second part); null: one */
+  const void* lambda;        /* ZS_MODE_LOGUP: the layer's batching challenge lam, one
+                                challenge-field element (host); null otherwise */
 } zs_prove_args;
 
 /* Caller-allocated host outputs. mlen = the message width of the wire form

@@ -16,7 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libzorch_amd.so"
 ZS_OK, ZS_EINVAL, ZS_ENOMEM, ZS_EHIP, ZS_EBUDGET, ZS_ENODEV = 0, 1, 2, 3, 4, 5
 ZS_F_BABYBEAR, ZS_F_BABYBEAR4, ZS_F_GOLDILOCKS = 1, 2, 3
 ZS_TR_SHA256, ZS_TR_BLAKE3 = 1, 2
-ZS_MODE_PRODUCT, ZS_MODE_EQ = 1, 2
+ZS_MODE_PRODUCT, ZS_MODE_EQ, ZS_MODE_LOGUP = 1, 2, 3
 ZS_MAX_ROUNDS = 31
 ZS_NOALLOC_SCRATCH_BYTES = 64 << 20
 ZS_WIRE_VALUES, ZS_WIRE_DOMAIN, ZS_WIRE_COEFFS, ZS_WIRE_COMPRESSED = 0, 1, 2, 3
@@ -90,6 +90,7 @@ class ProveArgsC(ctypes.Structure):
         ("claim", ctypes.c_void_p),
         ("wire", ctypes.POINTER(WireC)),
         ("eq_mass", ctypes.c_void_p),
+        ("lambda_", ctypes.c_void_p),
     ]
 
 
@@ -136,7 +137,7 @@ class ServerStatsC(ctypes.Structure):
 assert ctypes.sizeof(Sha256StateC) == 112
 assert ctypes.sizeof(Blake3StateC) == 632
 assert ctypes.sizeof(TranscriptC) == 648
-assert ctypes.sizeof(ProveArgsC) == 72
+assert ctypes.sizeof(ProveArgsC) == 80
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32

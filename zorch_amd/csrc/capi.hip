@@ -147,12 +147,28 @@ int validate_args(const zs_prove_args& a, const zs_transcript* t, uint32_t* npts
   const bool ok_pair = (a.table_field == a.chal_field) ||
                        (a.table_field == ZS_F_BABYBEAR && a.chal_field == ZS_F_BABYBEAR4);
   if (!ok_pair) return fail(ZS_EINVAL, "table field does not embed in the challenge field");
-  if (a.mode != ZS_MODE_PRODUCT && a.mode != ZS_MODE_EQ) return fail(ZS_EINVAL, "unknown mode");
-  if (a.degree < 1) return fail(ZS_EINVAL, "degree must be >= 1");
-  if (a.num_tables < 1 || a.num_tables > 4)
-    return fail(ZS_EINVAL, "between 1 and 4 stacked factor tables are supported");
-  if (a.degree != a.num_tables)
-    return fail(ZS_EINVAL, "the product summand's degree must equal the number of stacked tables");
+  if (a.mode != ZS_MODE_PRODUCT && a.mode != ZS_MODE_EQ && a.mode != ZS_MODE_LOGUP)
+    return fail(ZS_EINVAL, "unknown mode");
+  if (a.mode == ZS_MODE_LOGUP) {
+    // The LogUp summand (zorch/logup_gkr/prover.py:82-104): five factors,
+    // degree 3, one field, natural values on the wire.
+    if (a.num_tables != 5 || a.degree != 3)
+      return fail(ZS_EINVAL, "LogUp combine needs 5 factors [eq, n0, d1, n1, d0] of degree 3");
+    if (a.table_field != a.chal_field)
+      return fail(ZS_EINVAL, "LogUp tables must be in the challenge field");
+    if (!a.lambda) return fail(ZS_EINVAL, "a LogUp stage needs its batching challenge lambda");
+    if (a.eq_point || a.eq_mass) return fail(ZS_EINVAL, "a LogUp stage folds its eq table itself");
+    if (a.wire && a.wire->kind != ZS_WIRE_VALUES)
+      return fail(ZS_EINVAL, "a LogUp round sends its natural values");
+    if (a.refold_rounds) return fail(ZS_EINVAL, "refold_rounds must be 0 for a LogUp stage");
+  } else {
+    if (a.degree < 1) return fail(ZS_EINVAL, "degree must be >= 1");
+    if (a.num_tables < 1 || a.num_tables > 4)
+      return fail(ZS_EINVAL, "between 1 and 4 stacked factor tables are supported");
+    if (a.degree != a.num_tables)
+      return fail(ZS_EINVAL,
+                  "the product summand's degree must equal the number of stacked tables");
+  }
   // Table lengths of a round (2^(n - r) entries) are 32-bit in the kernels.
   if (a.rounds < 1 || a.rounds > ZS_MAX_ROUNDS)
     return fail(ZS_EINVAL, "rounds must be in [1, " + std::to_string(ZS_MAX_ROUNDS) + "]");
@@ -616,8 +632,10 @@ int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript, 
   if (rc) return rc;
   if ((rc = set_device(ctx))) return rc;
   std::unique_ptr<Instance> inst(
-      make_instance(args->table_field, args->chal_field, args->num_tables, npts,
-                    args->mode == ZS_MODE_EQ));
+      args->mode == ZS_MODE_LOGUP
+          ? make_logup_instance(args->chal_field)
+          : make_instance(args->table_field, args->chal_field, args->num_tables, npts,
+                          args->mode == ZS_MODE_EQ));
   if (!inst) return fail(ZS_EINVAL, "unsupported field / degree combination");
   if ((rc = inst->setup(ctx, *args, *transcript))) return rc;
   if ((rc = inst->enqueue(0, args->rounds, ctx->stream))) return rc;
@@ -640,6 +658,8 @@ int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
     if ((rc = validate_args(args[i], &transcripts[i], &npts))) return rc;
     if (args[i].refold_rounds)
       return fail(ZS_EINVAL, "batched proving folds every round (refold_rounds must be 0)");
+    if (args[i].mode == ZS_MODE_LOGUP)
+      return fail(ZS_EINVAL, "LogUp stages are proved one at a time (zs_prove)");
     const bool eq = args[i].mode == ZS_MODE_EQ;
     auto key = std::make_tuple(args[i].table_field, args[i].chal_field, args[i].num_tables, npts, eq);
     auto& g = groups[key];
@@ -674,8 +694,10 @@ int zs_state_create(zs_ctx* ctx, const zs_prove_args* args, zs_state** out) {
   if ((rc = set_device(ctx))) return rc;
   auto st = std::make_unique<zs_state>();
   st->ctx = ctx;
-  st->inst.reset(make_instance(args->table_field, args->chal_field, args->num_tables, npts,
-                               args->mode == ZS_MODE_EQ));
+  st->inst.reset(args->mode == ZS_MODE_LOGUP
+                     ? make_logup_instance(args->chal_field)
+                     : make_instance(args->table_field, args->chal_field, args->num_tables, npts,
+                                     args->mode == ZS_MODE_EQ));
   if (!st->inst) return fail(ZS_EINVAL, "unsupported field / degree combination");
   zs_transcript blank;
   memset(&blank, 0, sizeof(blank));

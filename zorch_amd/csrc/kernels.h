@@ -94,6 +94,7 @@ struct Ctl {
   typename FC::T wmat[MAXPTS][MAXPTS];
   typename FC::T cmat[MAXPTS][MAXPTS];
   typename FC::T cclaim[MAXPTS];
+  typename FC::T lam;  // LogUp stages: the layer's batching challenge
 };
 constexpr uint32_t WIRE_VALUES = 0, WIRE_DOMAIN = 1, WIRE_COEFFS = 2, WIRE_COMPRESSED = 3;
 

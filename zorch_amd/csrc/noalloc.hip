@@ -70,6 +70,8 @@ extern "C" int zs_prove_noalloc(zs_ctx* ctx, const zs_prove_args* args,
   if (rc) return rc;
   if (args->refold_rounds)
     return fail(ZS_EINVAL, "the no-allocation entry point folds every round (refold_rounds must be 0)");
+  if (args->mode == ZS_MODE_LOGUP)
+    return fail(ZS_EINVAL, "the no-allocation entry point proves product summands");
   if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
     return fail(ZS_EINVAL, "device index beyond the no-allocation scratch table");
   ZS_HIP(hipSetDevice(ctx->device));

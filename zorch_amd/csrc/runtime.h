@@ -212,6 +212,9 @@ class ServeKind {
   virtual void release(zs_ctx* ctx, ServeReq* r) = 0;
 };
 
+// LogUp stages (logup.hip): tables and challenges in `field`; nullptr for
+// an unsupported field.
+Instance* make_logup_instance(uint32_t field);
 // Factory: returns nullptr for an unsupported combination.
 Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
                         bool eq);

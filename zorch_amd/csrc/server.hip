@@ -504,6 +504,8 @@ int zs_server_submit(zs_server* S, const zs_prove_args* args, const zs_transcrip
   if (rc) return rc;
   if (args->refold_rounds)
     return fail(ZS_EINVAL, "served proofs fold every round (refold_rounds must be 0)");
+  if (args->mode == ZS_MODE_LOGUP)
+    return fail(ZS_EINVAL, "LogUp stages are proved one at a time (zs_prove)");
   const bool eq = args->mode == ZS_MODE_EQ;
   const uint32_t cb = field_bytes(args->chal_field);
   Pending p;

@@ -201,7 +201,8 @@ class DeviceFoldState:
     def __init__(self, tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
                  eq_point: FieldArray | None = None, ctx: Context | None = None, *,
                  wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None,
-                 eq_mass: FieldArray | None = None, refold_rounds: int = 0):
+                 eq_mass: FieldArray | None = None, refold_rounds: int = 0,
+                 lam: FieldArray | None = None):
         self.ctx = ctx or default_context()
         ptr, on_dev, keep, shape = tables_pointer(tables)
         self.k, width = shape
@@ -212,6 +213,7 @@ class DeviceFoldState:
         self.degree = degree
         self.npts = degree + 1 + (1 if mode == _lib.ZS_MODE_EQ else 0)
         claim_buf = el_buffer(claim, self.chal_field)
+        lam_buf = el_buffer(lam, self.chal_field) if lam is not None else None
         eq_buf = el_buffer(eq_point, self.chal_field) if eq_point is not None else None
         wire, wire_keep = wire_struct(wire_kind, domain, self.chal_field)
         mass_buf = el_buffer(eq_mass, self.chal_field) if eq_mass is not None else None
@@ -222,12 +224,13 @@ class DeviceFoldState:
             eq_point=eq_buf.ctypes.data if eq_buf is not None else None,
             claim=claim_buf.ctypes.data,
             wire=ctypes.pointer(wire) if wire is not None else None,
-            eq_mass=mass_buf.ctypes.data if mass_buf is not None else None)
+            eq_mass=mass_buf.ctypes.data if mass_buf is not None else None,
+            lambda_=lam_buf.ctypes.data if lam_buf is not None else None)
         self.mlen = message_width(args)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().zs_state_create(self.ctx.handle, ctypes.byref(args), ctypes.byref(h)))
         self._h = h
-        self._keep = keep
+        self._keep = (keep, lam_buf)
         self.bound = 0
         self.start_index = 0  # point index of this state's first round in its claim
         self.eq_point = eq_point

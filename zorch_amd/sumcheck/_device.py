@@ -49,10 +49,12 @@ class StageJob:
 
 def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
              eq_point: FieldArray | None = None, *, wire_kind: int = _lib.ZS_WIRE_VALUES,
-             domain: Any = None, refold_rounds: int = 0) -> StageJob:
+             domain: Any = None, refold_rounds: int = 0,
+             lam: FieldArray | None = None) -> StageJob:
     """Arguments and output buffers of one stage; `wire_kind` / `domain` pick
     the message wire form (natural values by default); `refold_rounds` first
-    rounds refold the untouched input (the sqrt-space engine)."""
+    rounds refold the untouched input (the sqrt-space engine); `lam` is a
+    LogUp stage's batching challenge."""
     chal = as_field(chal_field)
     ptr, on_dev, keep, shape = tables_pointer(tables)
     k, width = shape
@@ -64,12 +66,13 @@ def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
         raise ValueError(f"equality point needs {n} coordinates")
     wire, wire_keep = wire_struct(wire_kind, domain, chal)
     wire_ptr = ctypes.pointer(wire) if wire is not None else None
+    lam_buf = el_buffer(lam, chal) if lam is not None else None
     args = _lib.ProveArgsC(
         table_field=as_field(table_field).fid, chal_field=chal.fid, mode=mode, degree=degree,
         num_tables=k, rounds=n, tables=ptr, tables_on_device=1 if on_dev else 0,
         refold_rounds=refold_rounds,
         eq_point=eq_buf.ctypes.data if eq_buf is not None else None, claim=claim_buf.ctypes.data,
-        wire=wire_ptr)
+        wire=wire_ptr, lambda_=lam_buf.ctypes.data if lam_buf is not None else None)
     npts = message_width(args)
     msgs = np.zeros((n, npts, chal.words), dtype=chal.word_dtype)
     point = np.zeros((n, chal.words), dtype=chal.word_dtype)
@@ -77,7 +80,7 @@ def make_job(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int,
     final = np.zeros((k + 1, chal.words), dtype=chal.word_dtype)
     proof = _lib.ProofC(msgs=msgs.ctypes.data, point=point.ctypes.data, value=value.ctypes.data,
                         final_tables=final.ctypes.data)
-    return StageJob(args, proof, [keep, claim_buf, eq_buf, wire, wire_ptr, wire_keep], chal,
+    return StageJob(args, proof, [keep, claim_buf, eq_buf, wire, wire_ptr, wire_keep, lam_buf], chal,
                     npts, k, n, mode, msgs, point, value, final)
 
 
@@ -91,10 +94,10 @@ def _result(job: StageJob, st: _lib.TranscriptC, transcript) -> DeviceProof:
 def prove_stage(tables, claim: FieldArray, chal_field: Any, mode: int, degree: int, transcript,
                 eq_point: FieldArray | None = None, ctx: Context | None = None, *,
                 wire_kind: int = _lib.ZS_WIRE_VALUES, domain: Any = None,
-                refold_rounds: int = 0) -> DeviceProof:
+                refold_rounds: int = 0, lam: FieldArray | None = None) -> DeviceProof:
     ctx = ctx or default_context()
     job = make_job(tables, claim, chal_field, mode, degree, eq_point, wire_kind=wire_kind,
-                   domain=domain, refold_rounds=refold_rounds)
+                   domain=domain, refold_rounds=refold_rounds, lam=lam)
     st = transcript.state.copy()
     _lib.check(_lib.lib().zs_prove(ctx.handle, ctypes.byref(job.args), ctypes.byref(st),
                                    ctypes.byref(job.proof)))
