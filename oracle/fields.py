@@ -10,6 +10,9 @@ zorch/sumcheck/domain.py:101-102,232-233 and zorch/poly/univariate.py:55-65:
                       (limb order: zorch/utils/field.py:44-60).
 * ``goldilocks``      p = 2^64 - 2^32 + 1, canonical u64 storage
                       (reference docs/reference/conventions.md:266-268).
+* ``koalabear_mont``  p = 2^31 - 2^24 + 1, Montgomery words as BabyBear's;
+                      only the LogUp-GKR circuit's golden vectors use it
+                      (zorch/logup_gkr/testing/__init__.py:33,161-173).
 
 Elements are canonical Python ints (prime fields) or 4-tuples of canonical
 ints (the extension). ``to_words``/``from_words`` are the storage codecs; a
@@ -23,10 +26,11 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 BB_P = (1 << 31) - (1 << 27) + 1
+KB_P = (1 << 31) - (1 << 24) + 1
 GL_P = (1 << 64) - (1 << 32) + 1
 BB4_W = 11
 _R = 1 << 32
-_RINV_BB = pow(_R, -1, BB_P)
+_RINV = {BB_P: pow(_R, -1, BB_P), KB_P: pow(_R, -1, KB_P)}
 
 
 @dataclass(frozen=True)
@@ -86,7 +90,7 @@ class PrimeField:
         """Value of a storage word (reduced or not)."""
         (w,) = words
         if self.montgomery:
-            return (w * _RINV_BB) % self.p
+            return (w * _RINV[self.p]) % self.p
         return w % self.p
 
     def to_bytes(self, a: int) -> bytes:
@@ -188,6 +192,7 @@ class QuarticField:
 BABYBEAR = PrimeField("babybear_mont", BB_P, 4, True)
 GOLDILOCKS = PrimeField("goldilocks", GL_P, 8, False)
 BABYBEAR4 = QuarticField("babybearx4_mont", BABYBEAR, BB4_W)
+KOALABEAR = PrimeField("koalabear_mont", KB_P, 4, True)
 
 FIELDS = {f.name: f for f in (BABYBEAR, BABYBEAR4, GOLDILOCKS)}
 

@@ -8,6 +8,9 @@
 * ``LogupSumcheckRound`` under ``fold_rounds`` (``zs_state_round``): the
   messages and challenges equal the oracle's round by round on an arbitrary
   five-table state, and the folded tables read back equal the oracle's.
+* ``LogUpGkrProver`` / ``LogUpGkrVerifier`` (the stage pair): the input-layer
+  claim and the layer proofs equal the oracle chain's and the verifier
+  derives the same claim.
 * At 2^20 entries a layer (beyond the oracle's reach): the verifier accepts
   and the final carry is the input layer's MLEs at the final point, computed
   on the host (the reference roundtrip's closing check, verifier.py:12-15).
@@ -26,7 +29,9 @@ from oracle.sumcheck import fs_hop
 from oracle.transcript import OracleTranscript
 from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field, encode, from_storage, stack
-from zorch_amd.logup_gkr import (GkrLayer, LogupSumcheckRound, prove_gkr, verify_gkr)
+from zorch_amd.logup_gkr import (GkrLayer, LogUpGkrProver, LogUpGkrVerifier, LogUpOutputClaim,
+                                 LogupSumcheckRound, extract_outputs, build_pyramid, prove_gkr,
+                                 verify_gkr)
 from zorch_amd.logup_gkr.circuit import _interleave
 from zorch_amd.logup_gkr.prover import eval_mle
 from zorch_amd.prove import fold_rounds
@@ -92,6 +97,32 @@ def test_prove_gkr_matches_oracle(cf, tf, nb, nr):
     vcarry, t_ver, ok = verify_gkr(output, proofs, Sha256FieldTranscript.new(label, tf), ch)
     assert ok
     assert t_ver.stream_state() == t_dev.stream_state()
+
+
+@pytest.mark.parametrize("cf,tf", PAIRS)
+def test_stage_pair_matches_oracle(cf, tf):
+    F, T = OFIELD[cf], OFIELD[tf]
+    rng = random.Random(77)
+    nb, nr = 1, 4
+    olayer, dlayer = _layers(F, cf, rng, nb, nr)
+    ch = ChallengePolicy(cf)
+    out = extract_outputs(build_pyramid(dlayer)[-1])
+    claim = LogUpOutputClaim(out, nr)
+    res = LogUpGkrProver(ch).prove(claim, dlayer, Sha256FieldTranscript.new(b"stage", tf))
+    _, _, oproofs, ocarry, t_or = OL.prove_gkr(F, T, olayer, OracleTranscript.new(b"stage", T.item_bytes))
+    assert len(res.reduction_proof.layers) == nr == len(oproofs)
+    for p, q in zip(res.reduction_proof.layers, oproofs):
+        assert _words(p.round_polys) == _owords(F, [v for m in q.round_polys for v in m])
+    ic = res.reduced_claim
+    assert _words(ic.numerator) == _owords(F, [ocarry[0]])
+    assert _words(ic.denominator) == _owords(F, [ocarry[1]])
+    assert _words(ic.point) == _owords(F, ocarry[2])
+    assert res.transcript.stream_state() == tuple(t_or.state())
+    v = LogUpGkrVerifier(ch).verify(claim, res.reduction_proof,
+                                    Sha256FieldTranscript.new(b"stage", tf))
+    assert v.ok
+    assert _words(v.reduced_claim.numerator) == _words(ic.numerator)
+    assert _words(v.reduced_claim.point) == _words(ic.point)
 
 
 @pytest.mark.parametrize("cf,tf", PAIRS)

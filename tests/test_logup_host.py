@@ -14,7 +14,9 @@ from oracle import fields as OF
 from oracle import logup as OL
 from zorch_amd import _lib
 from zorch_amd.fields import FieldArray, as_field, from_storage, stack
-from zorch_amd.logup_gkr import (GkrLayer, LogupSummand, build_pyramid, extract_outputs,
+from zorch_amd.challenge import ChallengePolicy
+from zorch_amd.logup_gkr import (GkrLayer, GkrProof, LogUpGkrProver, LogUpGkrVerifier,
+                                 LogUpOutputClaim, LogupSummand, build_pyramid, extract_outputs,
                                  logup_combine)
 from zorch_amd.logup_gkr.prover import expand_eq_to_hypercube, eval_mle
 from zorch_amd.runtime import message_width
@@ -88,3 +90,18 @@ def test_host_circuit_and_combine_match_oracle(name):
     assert _words(eq) == _owords(F, eq_table(F, point))
     assert _words(eval_mle(_fa(F, name, vals[0][:8]), _fa(F, name, point))) == \
         _owords(F, [o_eval_mle(F, vals[0][:8], point)])
+
+
+def test_stage_rejects_a_layer_count_mismatch():
+    """The verifier owns the layer count: a witness or proof of another
+    height is rejected before any work (reference zorch/logup_gkr/stage.py:75-79,114-118)."""
+    F = OF.BABYBEAR4
+    rng = random.Random(2)
+    vals = [[F.from_int(rng.randrange(1 << 62)) for _ in range(16)] for _ in range(4)]
+    layer = GkrLayer(*(_fa(F, "babybearx4_mont", v) for v in vals), 1)
+    out = extract_outputs(build_pyramid(layer)[-1])
+    ch = ChallengePolicy("babybearx4_mont")
+    with pytest.raises(ValueError, match="claim expects 2 GKR layers, witness needs 3"):
+        LogUpGkrProver(ch).prove(LogUpOutputClaim(out, 2), layer, None)
+    with pytest.raises(ValueError, match="expected 3 GKR layers, got 2"):
+        LogUpGkrVerifier(ch).verify(LogUpOutputClaim(out, 3), GkrProof((None, None)), None)

@@ -5,6 +5,11 @@
   prover_test.py:75-87), and s(0) + s(1) is the hypercube sum (:89-94).
 * The circuit's output fractions add up to the input's (each transition
   preserves n0/d0 + n1/d1, checked with field inverses).
+* The circuit's outputs hash to the reference's golden vectors
+  (zorch/logup_gkr/testing/golden_vector_test.py:21-41): KoalaBear layers
+  drawn as the reference's ``random_first_layer(42, ...)`` draws them
+  (numpy PCG64 integers in [0, 2^30), seeds 42..45;
+  zorch/testkit/random_field.py:46-47), outputs as canonical u32 words.
 * prove -> verify accepts; the final carry is the input layer's interleaved
   numerator and denominator MLEs at the final point (the closing check of
   the reference's roundtrip, verifier.py:12-15); a tampered round polynomial
@@ -13,8 +18,10 @@
 
 from __future__ import annotations
 
+import hashlib
 import random
 
+import numpy as np
 import pytest
 
 from oracle import fields as OF
@@ -114,3 +121,43 @@ def test_tampering_is_rejected():
     bad = [OL.LayerProof(p.round_polys, p.point, p.n0, p.n1, p.d0, p.d1) for p in proofs]
     bad[0].n1 = F.add(bad[0].n1, F.one())
     assert not OL.verify_gkr(F, T, num, den, bad, t0)[2]
+
+
+# (batch variables, row variables) -> SHA-256 of the output MLEs' canonical
+# u32 words, numerator then denominator (golden_vector_test.py:21-27).
+_GOLDENS = {
+    (2, 3): "09f0bbf7f0cc0223e004a83f43c58f656463d2c157f32d627a182602da27fe93",
+    (2, 4): "a899f692b4ded58fd8218b37c29515ca3f3086588f5ddc6fa16e46e5e7ddb32b",
+    (2, 5): "a1568612f53abfbaed673c49fdfbfdec4fd321a04e7104a62cd0897a9a47413b",
+    (3, 4): "918a9179e246a562ee180d2e00f84febddfb53f3ca566de619800c5d06c535c2",
+}
+
+
+def _draw(seed, n):
+    return [int(v) for v in np.random.default_rng(seed).integers(0, 1 << 30, size=(n,),
+                                                                   dtype=np.int64)]
+
+
+@pytest.mark.parametrize("nb,nr", sorted(_GOLDENS))
+def test_circuit_outputs_match_reference_golden_vectors(nb, nr):
+    F = OF.KOALABEAR
+    w = 1 << (nb + nr)
+    first = OL.Layer(_draw(42, w), _draw(43, w), _draw(44, w), _draw(45, w), nb)
+    num, den = OL.extract_outputs(F, OL.build_pyramid(F, first)[-1])
+    words = np.array(list(num) + list(den), dtype="<u4").tobytes()
+    assert hashlib.sha256(words).hexdigest() == _GOLDENS[(nb, nr)]
+    # A planted slip in the transition (the cross term's second product
+    # dropped) moves the hash.
+    bad = OL.Layer(first.n0, first.n1, first.d0, first.d1, nb)
+    layers = [bad]
+    while layers[-1].num_row_variables > 0:
+        l = layers[-1]
+        h = len(l.n0) // 2
+        ev = lambda a: [a[2 * i] for i in range(h)]
+        od = lambda a: [a[2 * i + 1] for i in range(h)]
+        n = [F.mul(a, b) for a, b in zip(l.n0, l.d1)]
+        d = [F.mul(a, b) for a, b in zip(l.d0, l.d1)]
+        layers.append(OL.Layer(ev(n), od(n), ev(d), od(d), nb))
+    num, den = OL.extract_outputs(F, layers[-1])
+    words = np.array(list(num) + list(den), dtype="<u4").tobytes()
+    assert hashlib.sha256(words).hexdigest() != _GOLDENS[(nb, nr)]
