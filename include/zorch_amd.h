@@ -337,6 +337,42 @@ int zs_univariate_skip_round0(zs_ctx* ctx, const zs_prove_args* args, uint32_t s
 int zs_prove_small_value(zs_ctx* ctx, const zs_prove_args* args, uint32_t l0,
                          zs_transcript* transcript, zs_proof* out);
 
+/* ---- jagged LogUp-GKR layer --------------------------------------------
+ * Replaces prove_jagged_layer / the sumcheck of JaggedGkrLayerRound
+ * (zorch/logup_gkr/jagged_prover.py:124-448; round bodies
+ * zorch/logup_gkr/_jagged_rounds.py, zorch/sumcheck/jagged/rounds.py:36-85,
+ * fs.py:37-58). A jagged layer materialises row_counts[s] rows of segment s
+ * (batch-major, planes n0 | n1 | d0 | d1 of width = sum(row_counts) each)
+ * out of 2^row_variables virtual rows; the others are the neutral fraction
+ * (n = 0, d = 1). The sumcheck of sum_x eq(eval_point, x) (lam (n0 d1 +
+ * n1 d0) + d0 d1) over the virtual hypercube binds the row variables LSB
+ * first, then the batch variables; a round sends the four ascending
+ * coefficients of its cubic (interpolated through 0, 1, 1/2 and the root of
+ * the round's eq factor, the virtual rows' mass added in closed form).
+ * Planes, challenges and messages share `field`; the transcript is over
+ * `field` or its base field. */
+typedef struct zs_jagged_args {
+  uint32_t field;              /* ZS_F_* of planes, challenges and messages */
+  uint32_t num_segments;       /* 2^niv segments (a power of two) */
+  uint32_t row_variables;      /* nrv >= 1; niv + nrv <= ZS_MAX_ROUNDS */
+  uint32_t planes_on_device;   /* 1: `planes` is a device pointer */
+  const uint32_t* row_counts;  /* num_segments counts, 1 <= count <= 2^nrv (host) */
+  const void* planes;          /* 4 x sum(row_counts) elements: n0, n1, d0, d1 */
+  const void* eval_point;      /* niv + nrv elements, most significant first (host) */
+  const void* lambda;          /* the batching challenge, one element (host) */
+  const void* claim;           /* lam num_eval + den_eval, one element (host) */
+} zs_jagged_args;
+
+typedef struct zs_jagged_proof {
+  void* round_polys;  /* (niv + nrv) x 4 ascending coefficients */
+  void* challenges;   /* niv + nrv raw squeezed words, in sampling order (LSB first) */
+  void* openings;     /* 4 elements: n0, n1, d0, d1 at the bound point */
+  void* value;        /* the final reduced claim, one element */
+} zs_jagged_proof;
+
+int zs_prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args* args, zs_transcript* transcript,
+                          zs_jagged_proof* out);
+
 /* Many independent instances (uneven sizes, mixed kinds) in one submission;
  * instance i uses args[i], transcripts[i], outs[i], with the same results as
  * zs_prove on each. Instances of one kind (field pair, factor count, mode)

@@ -103,6 +103,29 @@ class ProofC(ctypes.Structure):
     ]
 
 
+class JaggedArgsC(ctypes.Structure):
+    _fields_ = [
+        ("field", ctypes.c_uint32),
+        ("num_segments", ctypes.c_uint32),
+        ("row_variables", ctypes.c_uint32),
+        ("planes_on_device", ctypes.c_uint32),
+        ("row_counts", ctypes.c_void_p),
+        ("planes", ctypes.c_void_p),
+        ("eval_point", ctypes.c_void_p),
+        ("lambda_", ctypes.c_void_p),
+        ("claim", ctypes.c_void_p),
+    ]
+
+
+class JaggedProofC(ctypes.Structure):
+    _fields_ = [
+        ("round_polys", ctypes.c_void_p),
+        ("challenges", ctypes.c_void_p),
+        ("openings", ctypes.c_void_p),
+        ("value", ctypes.c_void_p),
+    ]
+
+
 class KernelRecordC(ctypes.Structure):
     _fields_ = [
         ("round", ctypes.c_uint32),
@@ -138,6 +161,7 @@ assert ctypes.sizeof(Sha256StateC) == 112
 assert ctypes.sizeof(Blake3StateC) == 632
 assert ctypes.sizeof(TranscriptC) == 648
 assert ctypes.sizeof(ProveArgsC) == 80
+assert ctypes.sizeof(JaggedArgsC) == 56
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -186,6 +210,8 @@ SIGNATURES = {
                                          ctypes.POINTER(TranscriptC), _P, _P, _P, _P]),
     "zs_prove_small_value": (_INT, [_P, ctypes.POINTER(ProveArgsC), _U32,
                                     ctypes.POINTER(TranscriptC), ctypes.POINTER(ProofC)]),
+    "zs_prove_jagged_layer": (_INT, [_P, ctypes.POINTER(JaggedArgsC),
+                                     ctypes.POINTER(TranscriptC), ctypes.POINTER(JaggedProofC)]),
     "zs_state_create": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(_P)]),
     "zs_state_round": (_INT, [_P, ctypes.POINTER(TranscriptC), _P, _P, _P]),
     "zs_state_bound": (_INT, [_P, ctypes.POINTER(_U32)]),

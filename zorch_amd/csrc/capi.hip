@@ -644,6 +644,36 @@ int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript, 
   return inst->collect(out, transcript);
 }
 
+int zs_prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args* args, zs_transcript* transcript,
+                          zs_jagged_proof* out) {
+  if (!ctx || !args || !transcript || !out) return fail(ZS_EINVAL, "null argument");
+  const zs_jagged_args& a = *args;
+  if (!field_bytes(a.field)) return fail(ZS_EINVAL, "unknown field");
+  if (!a.row_counts || !a.planes || !a.eval_point || !a.lambda || !a.claim)
+    return fail(ZS_EINVAL, "null argument");
+  if (a.num_segments == 0 || (a.num_segments & (a.num_segments - 1)))
+    return fail(ZS_EINVAL, "the segment count must be a power of two");
+  uint32_t niv = 0;
+  while ((1u << niv) < a.num_segments) ++niv;
+  if (a.row_variables < 1)
+    return fail(ZS_EINVAL, "eval_point must carry at least one row variable");
+  if (a.row_variables + niv > ZS_MAX_ROUNDS) return fail(ZS_EINVAL, "too many variables");
+  for (uint32_t s = 0; s < a.num_segments; ++s)
+    if (a.row_counts[s] < 1 || a.row_counts[s] > (1u << a.row_variables))
+      return fail(ZS_EINVAL, "row count " + std::to_string(a.row_counts[s]) +
+                                 " outside [1, 2^" + std::to_string(a.row_variables) +
+                                 "]: the row-eq lookup would run out of bounds");
+  if (!known_kind(transcript->kind)) return fail(ZS_EINVAL, "unknown transcript kind");
+  const uint32_t tf = transcript->field;
+  if (!(tf == a.field || tf == base_of(a.field)))
+    return fail(ZS_EINVAL, "challenge field does not tile the transcript field");
+  if (transcript->item_bytes != field_bytes(tf))
+    return fail(ZS_EINVAL, "transcript item size mismatch");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  return prove_jagged_layer(ctx, a, transcript, out);
+}
+
 int zs_prove_batch(zs_ctx* ctx, size_t count, const zs_prove_args* args,
                    zs_transcript* transcripts, zs_proof* outs) {
   if (!ctx || (count && (!args || !transcripts || !outs))) return fail(ZS_EINVAL, "null argument");

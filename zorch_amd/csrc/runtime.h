@@ -215,6 +215,10 @@ class ServeKind {
 // LogUp stages (logup.hip): tables and challenges in `field`; nullptr for
 // an unsupported field.
 Instance* make_logup_instance(uint32_t field);
+// The jagged LogUp-GKR layer sumcheck (jagged.hip); arguments validated by
+// zs_prove_jagged_layer.
+int prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args& a, zs_transcript* t,
+                       zs_jagged_proof* out);
 // Factory: returns nullptr for an unsupported combination.
 Instance* make_instance(uint32_t table_field, uint32_t chal_field, uint32_t k, uint32_t npts,
                         bool eq);

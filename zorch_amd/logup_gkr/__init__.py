@@ -1,9 +1,16 @@
-"""Dense LogUp-GKR (drop-in for reference zorch/logup_gkr: circuit.py,
-prover.py, verifier.py, stage.py): the fractional-sum circuit on host field arrays, each
-layer's sumcheck on the GPU (``ZS_MODE_LOGUP`` stages of the C ABI)."""
+"""LogUp-GKR (drop-in for reference zorch/logup_gkr: circuit.py, prover.py,
+verifier.py, stage.py, jagged_prover.py, jagged_verifier.py): the
+fractional-sum circuit on host field arrays, each layer's sumcheck on the GPU
+(``ZS_MODE_LOGUP`` stages of the C ABI for dense layers,
+``zs_prove_jagged_layer`` for jagged ones)."""
 
-from zorch_amd.logup_gkr.circuit import (GkrLayer, LogUpGkrOutput, build_pyramid,
-                                         extract_outputs, layer_transition)
+from zorch_amd.logup_gkr.circuit import (GkrLayer, JaggedGkrLayer, LogUpGkrOutput,
+                                         build_jagged_pyramid, build_pyramid,
+                                         extract_jagged_outputs, extract_outputs,
+                                         jagged_layer_transition, layer_transition)
+from zorch_amd.logup_gkr.jagged_prover import (JaggedGkrLayerRound, JaggedLayerProof,
+                                               RoundWidthCaps, prove_jagged_layer)
+from zorch_amd.logup_gkr.jagged_verifier import JaggedGkrLayerRound as JaggedVerifierLayerRound
 from zorch_amd.logup_gkr.prover import (GkrLayerRound, LayerProof, LogupSumcheckRound,
                                         LogupSummand, bind_output, fold_carry, logup_combine,
                                         prove_gkr)
@@ -16,4 +23,6 @@ __all__ = ["GkrLayer", "LogUpGkrOutput", "build_pyramid", "extract_outputs", "la
            "GkrLayerRound", "LayerProof", "LogupSumcheckRound", "LogupSummand", "bind_output",
            "fold_carry", "logup_combine", "prove_gkr", "GkrLayerVerifierRound", "verify_gkr",
            "GkrProof", "InputLayerClaim", "LogUpGkrProver", "LogUpGkrVerifier",
-           "LogUpOutputClaim"]
+           "LogUpOutputClaim", "JaggedGkrLayer", "jagged_layer_transition",
+           "build_jagged_pyramid", "extract_jagged_outputs", "JaggedGkrLayerRound",
+           "JaggedLayerProof", "RoundWidthCaps", "prove_jagged_layer", "JaggedVerifierLayerRound"]

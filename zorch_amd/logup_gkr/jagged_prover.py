@@ -1,0 +1,145 @@
+"""Jagged LogUp-GKR prover (drop-in for reference zorch/logup_gkr/jagged_prover.py).
+
+A jagged layer materialises only sum(row_counts) of its virtual
+2^(niv + nrv) positions; the others hold the neutral fraction (n = 0,
+d = 1), whose LogUp summand is its eq weight alone. The layer's sumcheck
+binds the row variables LSB first, then the batch variables, and sends each
+round's cubic as four ascending coefficients, interpolated through 0, 1,
+1/2 and the root of the round's eq factor with the virtual rows' mass added
+in closed form (jagged_prover.py:1-33). The whole layer sumcheck is one
+device call (``zs_prove_jagged_layer``: one fold-and-sum launch per row
+round with the finisher block, one block for the batch rounds);
+``JaggedGkrLayerRound`` adds the carry around it as the dense
+``GkrLayerRound`` does (jagged_prover.py:453-548, 671-712).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from zorch_amd import _lib
+from zorch_amd.challenge import ChallengePolicy
+from zorch_amd.fields import FieldArray, as_field, stack
+from zorch_amd.logup_gkr.circuit import JaggedGkrLayer
+from zorch_amd.logup_gkr.prover import fold_carry
+from zorch_amd.runtime import default_context, el_buffer
+from zorch_amd.sumcheck.jagged.types import RoundWidthCaps
+
+DEGREE = 3
+
+
+@dataclass(frozen=True)
+class JaggedLayerProof:
+    """One jagged layer's sumcheck transcript: the batching challenge and
+    the claim the layer entered with, the coefficient-form round polynomials,
+    the bound point (most significant first: the challenges reversed) and
+    the four openings (jagged_prover.py:89-121)."""
+
+    lam: FieldArray
+    claim: FieldArray
+    round_polys: FieldArray  # (niv + nrv, 4) ascending coefficients
+    point: FieldArray
+    numerator_0: FieldArray
+    numerator_1: FieldArray
+    denominator_0: FieldArray
+    denominator_1: FieldArray
+
+
+def _device_prove(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
+                  eval_point: FieldArray, transcript, chal, ctx=None):
+    ctx = ctx or default_context()
+    niv = layer.num_batch_variables
+    nrv = eval_point.shape[0] - niv
+    live = layer.live
+    planes = stack([a[:live].astype(chal) for a in (layer.numerator_0, layer.numerator_1,
+                                                    layer.denominator_0, layer.denominator_1)])
+    words = np.ascontiguousarray(planes.normalized().words)
+    counts = np.ascontiguousarray(np.asarray(layer.row_counts, dtype=np.uint32))
+    pt = el_buffer(eval_point, chal)
+    lam_b = el_buffer(lam, chal)
+    claim_b = el_buffer(claim, chal)
+    R = niv + nrv
+    polys = np.zeros((R, 4, chal.words), dtype=chal.word_dtype)
+    chals = np.zeros((R, chal.words), dtype=chal.word_dtype)
+    opens = np.zeros((4, chal.words), dtype=chal.word_dtype)
+    value = np.zeros((chal.words,), dtype=chal.word_dtype)
+    args = _lib.JaggedArgsC(field=chal.fid, num_segments=layer.num_batches, row_variables=nrv,
+                            planes_on_device=0, row_counts=counts.ctypes.data,
+                            planes=words.ctypes.data, eval_point=pt.ctypes.data,
+                            lambda_=lam_b.ctypes.data, claim=claim_b.ctypes.data)
+    out = _lib.JaggedProofC(round_polys=polys.ctypes.data, challenges=chals.ctypes.data,
+                            openings=opens.ctypes.data, value=value.ctypes.data)
+    st = transcript.state.copy()
+    _lib.check(_lib.lib().zs_prove_jagged_layer(ctx.handle, ctypes.byref(args), ctypes.byref(st),
+                                                ctypes.byref(out)))
+    return (FieldArray(polys, chal), FieldArray(chals, chal).normalized(), FieldArray(opens, chal),
+            type(transcript)(st, transcript.dtype))
+
+
+def _check_layer(layer: JaggedGkrLayer, eval_point: FieldArray, caps) -> int:
+    if caps is None:
+        raise ValueError("a jagged layer proves under caps; pass RoundWidthCaps")
+    niv = layer.num_batch_variables
+    nrv = eval_point.shape[0] - niv
+    if nrv < 1:
+        raise ValueError(f"eval_point must carry at least one row variable: got "
+                         f"{eval_point.shape[0]} coordinates for {niv} batch variables")
+    if max(layer.row_counts) > (1 << nrv):
+        raise ValueError(f"row count {max(layer.row_counts)} exceeds the virtual row space "
+                         f"2^{nrv}; the row-eq lookup would run out of bounds")
+    if min(layer.row_counts) < 1:
+        raise ValueError("every segment needs at least one row")
+    caps.check(layer.width, nrv, niv)
+    return nrv
+
+
+def prove_jagged_layer(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
+                       eval_point: FieldArray, transcript, *, challenges: ChallengePolicy,
+                       caps: RoundWidthCaps | None = None):
+    """One jagged layer's sumcheck from an explicit lam and claim (the
+    standalone seam, jagged_prover.py:124-183). `eval_point` is most
+    significant first over (batch || row) variables. Returns (bound point,
+    transcript, proof)."""
+    _check_layer(layer, eval_point, caps)
+    chal = as_field(challenges.dtype)
+    lam = lam.astype(chal).normalized()
+    claim = claim.astype(chal).normalized()
+    eval_point = eval_point.astype(chal)
+    polys, chals, opens, transcript = _device_prove(layer, lam, claim, eval_point, transcript,
+                                                    chal)
+    point = chals[::-1]
+    proof = JaggedLayerProof(lam, claim, polys, point, opens[0], opens[1], opens[2], opens[3])
+    return point, transcript, proof
+
+
+class JaggedGkrLayerRound:
+    """Prove one jagged layer with the dense chain's carry: sample lam, form
+    lam num_eval + den_eval, prove, observe the openings, sample the child
+    selector and fold the carry (jagged_prover.py:453-548, 671-712)."""
+
+    def __init__(self, layer: JaggedGkrLayer, challenges: ChallengePolicy, *,
+                 caps: RoundWidthCaps | None = None, layer_bufs=None) -> None:
+        self.layer = layer
+        self.challenges = challenges
+        self.caps = caps
+
+    def __call__(self, claim, transcript):
+        num_eval, den_eval, eval_point = claim
+        if self.caps is None:
+            raise ValueError("a jagged layer proves under caps; pass RoundWidthCaps")
+        transcript, lam = self.challenges.sample(transcript)
+        lam = lam.normalized()
+        batched = lam * num_eval + den_eval
+        point, transcript, proof = prove_jagged_layer(
+            self.layer, lam, batched, eval_point, transcript, challenges=self.challenges,
+            caps=self.caps)
+        n0, n1 = proof.numerator_0, proof.numerator_1
+        d0, d1 = proof.denominator_0, proof.denominator_1
+        transcript, r = self.challenges.observe_and_sample(transcript, stack([n0, n1, d0, d1]))
+        return fold_carry(n0, n1, d0, d1, point, r.normalized()), transcript, proof
+
+
+__all__ = ["JaggedLayerProof", "prove_jagged_layer", "JaggedGkrLayerRound", "RoundWidthCaps"]
