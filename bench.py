@@ -18,10 +18,13 @@ The metric has three parts, all in the one JSON line a default run prints:
   n ~ U{14..24}) through the continuous-batching server, p50/p99 latency and
   proofs/s, against one proof per call;
 * ``engines``: the sqrt-space, univariate-skip and small-value engines at
-  n = 24 against the dense prover on the same tables.
+  n = 24 against the dense prover on the same tables;
+* ``logup``: a dense LogUp-GKR layer sumcheck (2^22 rows of five tables)
+  and a jagged one (256 segments, row counts ~ U{1..2^14}), BabyBear^4.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
-                    [--no-batch] [--no-serve] [--no-cpu-baseline]
+                    [--no-batch] [--no-serve] [--no-cpu-baseline] [--no-engines]
+                    [--no-logup]
 """
 
 from __future__ import annotations
@@ -665,6 +668,89 @@ def bench_engines(args, n: int = 24) -> dict:
     return out
 
 
+def bench_logup(args, n_dense: int = 22, nseg: int = 256, nrv: int = 14) -> dict:
+    """LogUp-GKR layer sumchecks on resident BabyBear^4 tables (SHA-256 over
+    BabyBear): a dense layer of 2^n_dense rows (five tables [eq, n0, d1, n1,
+    d0], one ZS_MODE_LOGUP stage) and a jagged layer of `nseg` segments with
+    row counts ~ U{1..2^nrv} (four planes, zs_prove_jagged_layer). Whole layer
+    from the host call to the returned proof, and each kernel with HIP events
+    (algorithmic bytes: every live entry read once a round, folded entries
+    written once)."""
+    from zorch_amd import _lib
+    from zorch_amd.challenge import ChallengePolicy
+    from zorch_amd.fields import FieldArray, babybear_mont, babybearx4_mont
+    from zorch_amd.logup_gkr import JaggedGkrLayer, RoundWidthCaps
+    from zorch_amd.logup_gkr.jagged_prover import _device_prove
+    from zorch_amd.runtime import DeviceTables, default_context, profile, profile_read
+    from zorch_amd.sha256_field_transcript import Sha256FieldTranscript
+    from zorch_amd.sumcheck._device import prove_stage
+
+    ctx = default_context()
+    f = babybearx4_mont
+    rng = np.random.default_rng(17)
+    reps = max(3, min(args.steps, 10))
+
+    def rand(*shape):
+        return FieldArray(rng.integers(0, f.p, size=shape + (4,), dtype=np.uint64)
+                          .astype(np.uint32), f)
+
+    def timed(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    def kernels(fn, kinds):
+        profile(ctx, True)
+        fn()
+        recs = [r for r in profile_read(ctx) if r["kind"] in kinds]
+        profile(ctx, False)
+        return recs
+
+    def summary(recs):
+        top = max(recs, key=lambda r: r["bytes"])
+        return {"kernels": len(recs), "kernel_ms": sum(r["ms"] for r in recs),
+                "dominant": {"round": top["round"], "bytes": top["bytes"], "ms": top["ms"],
+                             "gbs": top["bytes"] / (top["ms"] * 1e-3) / 1e9,
+                             "frac": top["bytes"] / (top["ms"] * 1e-3) / 8e12},
+                "per_round_gbs": [round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1) for r in recs]}
+
+    out = {"field": "babybearx4_mont", "transcript": "sha256 over babybear_mont", "reps": reps}
+    # dense layer
+    N = 1 << n_dense
+    state = DeviceTables.from_host(rand(5, N), ctx)
+    lam, claim = rand(), rand()
+    dense = lambda: prove_stage(state, claim, f, _lib.ZS_MODE_LOGUP, 3,
+                                Sha256FieldTranscript.new(b"logup", babybear_mont), lam=lam)
+    ms = timed(dense)
+    out["dense"] = {"rows": N, "tables": 5, "ms_per_layer": ms,
+                    "table_elems_per_s": 5 * N / ms * 1e3, **summary(kernels(dense, {"logup"}))}
+    state.free()
+    # jagged layer
+    counts = tuple(int(c) for c in rng.integers(1, (1 << nrv) + 1, size=nseg))
+    live = sum(counts)
+    planes = rand(4, live)
+    layer = JaggedGkrLayer(*(planes[q] for q in range(4)), counts)
+    resident = DeviceTables.from_host(planes, ctx)
+    niv = nseg.bit_length() - 1
+    z = rand(niv + nrv)
+    ch = ChallengePolicy(f)
+    caps = RoundWidthCaps(elements=live + (-live % 4) + 4 * nseg, eq_row=1 << nrv,
+                          interaction=max(4, nseg))
+    caps.check(layer.width, nrv, niv)
+    jag = lambda: _device_prove(layer, lam, claim, z, Sha256FieldTranscript.new(b"jagged",
+                                                                                babybear_mont),
+                                f, ctx, resident=resident)
+    ms = timed(jag)
+    out["jagged"] = {"segments": nseg, "row_variables": nrv, "live_rows": live,
+                     "virtual_rows": nseg << nrv, "ms_per_layer": ms,
+                     "live_elems_per_s": 4 * live / ms * 1e3,
+                     **summary(kernels(jag, {"jagged", "jagged_batch"}))}
+    resident.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -676,6 +762,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="leave out the served part")
     ap.add_argument("--no-engines", action="store_true",
                     help="leave out the sqrt-space / univariate-skip / small-value part")
+    ap.add_argument("--no-logup", action="store_true",
+                    help="leave out the LogUp-GKR layer part")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
@@ -730,6 +818,9 @@ def main() -> int:
     if full and not args.no_engines:
         out["engines"] = bench_engines(args)
         print("engines done", file=sys.stderr, flush=True)
+    if full and not args.no_logup:
+        out["logup"] = bench_logup(args)
+        print("logup done", file=sys.stderr, flush=True)
     if batch_insts is not None:
         out["batch"] = bench_batch(args, batch_insts)
         del batch_insts

@@ -194,7 +194,10 @@ int zs_hbm_bandwidth(zs_ctx* ctx, size_t bytes, uint32_t reps, double* read_gbs,
  * context launches is bracketed by HIP events on the context's stream. A read
  * returns one record per kernel with its algorithmic HBM bytes (fused-pass
  * model) and its measured duration, then clears the list. kind: 1 = pass,
- * 2 = finisher, 3 = tail (all remaining rounds in one block). */
+ * 2 = finisher, 3 = tail (all remaining rounds in one block), 4 = batch step,
+ * 5 = chain, 6-8 = the look-ahead pass, chain and tail, 9 = rounds 0-1 from
+ * the input, 10 = round 2 from the input, 11 = refold round, 12 = engine
+ * pass, 13 = LogUp round, 14 = jagged row round, 15 = jagged batch rounds. */
 typedef struct zs_kernel_record {
   uint32_t round;
   uint32_t kind;

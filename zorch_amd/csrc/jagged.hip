@@ -492,20 +492,43 @@ static int prove_jagged_typed(zs_ctx* ctx, const zs_jagged_args& a, zs_transcrip
   jagged_eq_kernel<F><<<eq_grid(((size_t)2 << niv) - 1), PASS_THREADS, 0, s>>>(g.zint, niv,
                                                                                   (T*)g.eint);
   ZS_HIP(hipGetLastError());
+  // Kernel timing (zs_ctx_profile): algorithmic bytes of the four planes.
+  auto begin = [&]() -> hipEvent_t {
+    if (!ctx->profile) return nullptr;
+    hipEvent_t e = ctx->event();
+    (void)hipEventRecord(e, s);
+    return e;
+  };
+  auto mark = [&](hipEvent_t e, uint32_t round, uint32_t kind, uint32_t grid, uint64_t bytes) {
+    if (!e) return;
+    hipEvent_t stop = ctx->event();
+    (void)hipEventRecord(stop, s);
+    ctx->records.push_back(KernelRecord{round, kind, grid, bytes, e, stop});
+  };
+  uint64_t live_prev = width;
   const uint32_t maxg = ctx->tune.max_grid < (uint32_t)MAXG ? ctx->tune.max_grid : (uint32_t)MAXG;
   for (uint32_t k = 0; k < nrv; ++k) {
-    const uint32_t P = segs[(size_t)k * nseg + nseg - 1].cum_pairs;
+    const JagSeg& last = segs[(size_t)k * nseg + nseg - 1];
+    const uint32_t P = last.cum_pairs;
+    const uint64_t live = (uint64_t)last.base + last.count;
     size_t grid = (P + PASS_THREADS - 1) / PASS_THREADS;
     if (grid > maxg) grid = maxg;
     if (grid < 1) grid = 1;
+    hipEvent_t ev = begin();
     if (k == 0)
       jagged_row_kernel<F, false><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(g, k);
     else
       jagged_row_kernel<F, true><<<(uint32_t)grid + 1, PASS_THREADS, 0, s>>>(g, k);
     ZS_HIP(hipGetLastError());
+    // Round 0 reads the live rows; round k reads round k-1's and writes its own.
+    mark(ev, k, KREC_JAGGED, (uint32_t)grid,
+         (uint64_t)4 * sizeof(T) * (k == 0 ? live : live_prev + live));
+    live_prev = live;
   }
+  hipEvent_t evb = begin();
   jagged_batch_kernel<F><<<1, PASS_THREADS, 0, s>>>(g);
   ZS_HIP(hipGetLastError());
+  mark(evb, nrv, KREC_JAGGED_BATCH, 1, (uint64_t)4 * sizeof(T) * (live_prev + 2 * nseg));
   ZS_HIP(hipMemcpyAsync(h, b, out_bytes, hipMemcpyDeviceToHost, s));
   ZS_HIP(hipStreamSynchronize(s));
   const JagCtl<F>& jo = *(const JagCtl<F>*)(h + o_ctl);

@@ -182,6 +182,11 @@ class LogupInstance final : public Instance {
       if (grid < 1) grid = 1;
       const uint32_t blocks = (uint32_t)grid + 1;
       const uint32_t need_claim = r + 1 == last ? 1u : 0u;  // the claim is read after `last`
+      hipEvent_t ev = nullptr;
+      if (ctx_->profile) {
+        ev = ctx_->event();
+        (void)hipEventRecord(ev, s);
+      }
       if (r == 0) {
         logup_pass_kernel<F, false>
             <<<blocks, PASS_THREADS, 0, s>>>(args_, args_.in, args_.in_stride, L, r, need_claim);
@@ -194,6 +199,15 @@ class LogupInstance final : public Instance {
                                                                     need_claim);
       }
       ZS_HIP(hipGetLastError());
+      if (ev) {
+        // Five tables: round 0 reads L entries each; later rounds read the
+        // previous round's 2L and write L.
+        const uint64_t w = sizeof(T);
+        const uint64_t b = r == 0 ? (uint64_t)LG_TABLES * L * w : (uint64_t)LG_TABLES * 3 * L * w;
+        hipEvent_t stop = ctx_->event();
+        (void)hipEventRecord(stop, s);
+        ctx_->records.push_back(KernelRecord{r, KREC_LOGUP, (uint32_t)grid, b, ev, stop});
+      }
     }
     return ZS_OK;
   }

@@ -94,7 +94,8 @@ enum {
   KREC_PASS = 1, KREC_FINISH = 2, KREC_TAIL = 3, KREC_BATCH = 4, KREC_CHAIN = 5,
   KREC_PASS_LA = 6, KREC_CHAIN_LA = 7, KREC_TAIL_LA = 8,  // look-ahead variants
   KREC_FUSED01 = 9, KREC_FOLD2 = 10,                      // rounds 0-2 from the input
-  KREC_REFOLD = 11, KREC_ENGINE = 12                      // sqrt-space rounds; other engines
+  KREC_REFOLD = 11, KREC_ENGINE = 12,                     // sqrt-space rounds; other engines
+  KREC_LOGUP = 13, KREC_JAGGED = 14, KREC_JAGGED_BATCH = 15  // LogUp-GKR layer rounds
 };
 
 }  // namespace zs

@@ -49,14 +49,23 @@ class JaggedLayerProof:
 
 
 def _device_prove(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
-                  eval_point: FieldArray, transcript, chal, ctx=None):
+                  eval_point: FieldArray, transcript, chal, ctx=None, resident=None):
+    """`resident`: the four live planes already in HBM (DeviceTables of
+    shape (4, live) in the challenge field) instead of the layer's host
+    arrays."""
     ctx = ctx or default_context()
     niv = layer.num_batch_variables
     nrv = eval_point.shape[0] - niv
     live = layer.live
-    planes = stack([a[:live].astype(chal) for a in (layer.numerator_0, layer.numerator_1,
-                                                    layer.denominator_0, layer.denominator_1)])
-    words = np.ascontiguousarray(planes.normalized().words)
+    if resident is not None:
+        if tuple(resident.shape) != (4, live) or resident.dtype != chal:
+            raise ValueError(f"resident planes must be (4, {live}) in {chal}")
+        words, on_dev, ptr = None, 1, resident.data_ptr()
+    else:
+        planes = stack([a[:live].astype(chal) for a in (layer.numerator_0, layer.numerator_1,
+                                                        layer.denominator_0, layer.denominator_1)])
+        words = np.ascontiguousarray(planes.normalized().words)
+        on_dev, ptr = 0, words.ctypes.data
     counts = np.ascontiguousarray(np.asarray(layer.row_counts, dtype=np.uint32))
     pt = el_buffer(eval_point, chal)
     lam_b = el_buffer(lam, chal)
@@ -67,8 +76,8 @@ def _device_prove(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
     opens = np.zeros((4, chal.words), dtype=chal.word_dtype)
     value = np.zeros((chal.words,), dtype=chal.word_dtype)
     args = _lib.JaggedArgsC(field=chal.fid, num_segments=layer.num_batches, row_variables=nrv,
-                            planes_on_device=0, row_counts=counts.ctypes.data,
-                            planes=words.ctypes.data, eval_point=pt.ctypes.data,
+                            planes_on_device=on_dev, row_counts=counts.ctypes.data,
+                            planes=ptr, eval_point=pt.ctypes.data,
                             lambda_=lam_b.ctypes.data, claim=claim_b.ctypes.data)
     out = _lib.JaggedProofC(round_polys=polys.ctypes.data, challenges=chals.ctypes.data,
                             openings=opens.ctypes.data, value=value.ctypes.data)

@@ -284,7 +284,8 @@ def profile_read(ctx: Context, cap: int = 1 << 16) -> list[dict]:
     n = ctypes.c_uint32(0)
     _lib.check(_lib.lib().zs_ctx_profile_read(ctx.handle, recs, cap, ctypes.byref(n)))
     kinds = {1: "pass", 2: "finish", 3: "tail", 4: "batch", 5: "chain", 6: "pass_la",
-             7: "chain_la", 8: "tail_la", 9: "fused01", 10: "fold2", 11: "refold", 12: "engine"}
+             7: "chain_la", 8: "tail_la", 9: "fused01", 10: "fold2", 11: "refold", 12: "engine",
+             13: "logup", 14: "jagged", 15: "jagged_batch"}
     return [{"round": r.round, "kind": kinds.get(r.kind, str(r.kind)), "grid": r.grid,
              "bytes": int(r.bytes), "ms": float(r.ms)} for r in recs[: n.value]]
 
