@@ -725,7 +725,7 @@ def bench_logup(args, n_dense: int = 22, nseg: int = 256, nrv: int = 14) -> dict
                                 Sha256FieldTranscript.new(b"logup", babybear_mont), lam=lam)
     ms = timed(dense)
     out["dense"] = {"rows": N, "tables": 5, "ms_per_layer": ms,
-                    "table_elems_per_s": 5 * N / ms * 1e3, **summary(kernels(dense, {"logup"}))}
+                    "table_elems_per_s": 5 * N / ms * 1e3, **summary(kernels(dense, {"logup", "tail"}))}
     state.free()
     # jagged layer
     counts = tuple(int(c) for c in rng.integers(1, (1 << nrv) + 1, size=nseg))

@@ -4,8 +4,8 @@
 * ``prove_jagged_layer``: the round polynomials, the sampled challenges, the
   openings and the transcript state equal the oracle's bit for bit, for three
   field pairs, over layouts with odd, saturated and even segments, one
-  segment (no batch variables), a one-row-variable layer and a wide random
-  layout.
+  segment (no batch variables), a one-row-variable layer, a wide random
+  layout and a deep one whose first round runs multi-block.
 * ``JaggedGkrLayerRound`` over a jagged pyramid: every layer proof and the
   final carry equal the oracle chain's; the jagged verifier accepts and
   reaches the same carry.
@@ -70,13 +70,17 @@ def _layers(F, field, rng, counts):
             JaggedGkrLayer(*(_fa(F, field, v) for v in vals), tuple(counts)))
 
 
+# Rounds of more than 2048 pairs run as multi-block launches, the rest in one
+# block: "deep" (8 segments of 512..1024 rows) takes both.
 LAYOUTS = [((3, 1, 5, 2), 3), ((1, 1, 1, 1), 3), ((7,), 3), ((2, 4, 1, 3, 8, 6, 5, 1), 4),
-           ((2, 1), 1), ("wide", 7)]
+           ((2, 1), 1), ("wide", 7), ("deep", 10)]
 
 
 def _counts(spec, nrv, rng):
     if spec == "wide":
         return tuple(rng.randrange(1, (1 << nrv) + 1) for _ in range(32))
+    if spec == "deep":
+        return tuple(rng.randrange(1 << (nrv - 1), (1 << nrv) + 1) for _ in range(8))
     return spec
 
 

@@ -99,6 +99,22 @@ def test_prove_gkr_matches_oracle(cf, tf, nb, nr):
     assert t_ver.stream_state() == t_dev.stream_state()
 
 
+def test_multi_block_round_matches_oracle():
+    """A layer of 2^13 entries: round 0 (4096 pairs) runs as a multi-block
+    launch, the later rounds in one block."""
+    cf, tf = PAIRS[0]
+    F, T = OFIELD[cf], OFIELD[tf]
+    rng = random.Random(13)
+    olayer, dlayer = _layers(F, cf, rng, 0, 13)
+    ch = ChallengePolicy(cf)
+    _, output, proofs, carry, t_dev = prove_gkr(dlayer, Sha256FieldTranscript.new(b"mb", tf), ch)
+    _, _, oproofs, ocarry, t_or = OL.prove_gkr(F, T, olayer, OracleTranscript.new(b"mb", T.item_bytes))
+    for p, q in zip(proofs, oproofs):
+        assert _words(p.round_polys) == _owords(F, [v for m in q.round_polys for v in m])
+    assert _words(carry[2]) == _owords(F, ocarry[2])
+    assert t_dev.stream_state() == tuple(t_or.state())
+
+
 @pytest.mark.parametrize("cf,tf", PAIRS)
 def test_stage_pair_matches_oracle(cf, tf):
     F, T = OFIELD[cf], OFIELD[tf]

@@ -28,6 +28,8 @@
 namespace zs {
 
 constexpr int JG_NS = 3;  // sums a round: ez, eh, es
+// Row rounds with at most this many pairs run in the one-block kernel.
+constexpr uint32_t JG_TAIL_PAIRS = 2048;
 
 // Row round k, segment s.
 struct JagSeg {
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_eq_kernel(const typename 
 // The finisher of round `round` (wave 0): message from the sums `s` (LDS),
 // hop, challenge, claim and eq masses. The proof state is in `sm`, `pad` and
 // `eq_adj` (every lane holds the same values).
-template <class F>
+template <class F, int HOP>
 __device__ __forceinline__ void jag_finish(const JagArgs<F>& a, uint32_t round,
                                            const typename F::T* s, FinishLds<F>& sm,
                                            typename F::T& pad, typename F::T& eq_adj) {
@@ -133,8 +135,12 @@ __device__ __forceinline__ void jag_finish(const JagArgs<F>& a, uint32_t round,
   }
   wave_sync();
   const Ctl<F>* ctl = &a.ctl->c;
-  fs_hop<HOP_PASS>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
-                   4u * (uint32_t)EB, ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+  if constexpr (HOP == HOP_INLINE)
+    fs_hop_body(sm, reinterpret_cast<const uint8_t*>(sm.msg), 4u * (uint32_t)EB, ctl->item_bytes,
+                ctl->limbs, ctl->tr_kind);
+  else
+    fs_hop<HOP>((ZS_LDS HopLds*)static_cast<HopLds*>(&sm), (const ZS_LDS uint8_t*)sm.msg,
+                4u * (uint32_t)EB, ctl->item_bytes, ctl->limbs, ctl->tr_kind);
   if (lane < (uint32_t)EB) a.point[(size_t)round * EB + lane] = sm.out[lane];
   const T r = el_from_words<F>(reinterpret_cast<const uint32_t*>(sm.out));
   T claim = sm.msg[3];
@@ -166,41 +172,16 @@ struct JagLds {
   typename F::T pad, eq_adj;
 };
 
-// Row round k as 1 + nblk blocks (see the file comment).
+// Pairs p = start, start + stride, ... of row round k: fold (FOLD: round
+// k-1's rows at r into round k's state) and add W B(even), W B(even + odd)
+// and W to acc.
 template <class F, bool FOLD>
-__global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, uint32_t k) {
+__device__ __forceinline__ void jag_row_pairs(const JagArgs<F>& a, uint32_t k, typename F::T lam,
+                                              typename F::T r, uint32_t start, uint32_t stride,
+                                              typename F::T (&acc)[JG_NS]) {
   using T = typename F::T;
-  __shared__ JagLds<F> L;
-  auto& sh = L.round;
-  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
-  JagCtl<F>* jc = a.ctl;
-  Ctl<F>* ctl = &jc->c;
-  if (blockIdx.x == 0) {
-    if (tid < 64) fin_load(sh.sm, ctl);
-    if (tid == 0) {
-      sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
-      L.pad = jc->pad;
-      L.eq_adj = jc->eq_adj;
-    }
-    __syncthreads();
-    if (!sh.is_last) return;
-    slot_rows_sum<F, JG_NS>(a.slots, sh);
-    if (tid < 64) {
-      T pad = L.pad, eq_adj = L.eq_adj;
-      jag_finish<F>(a, k, sh.out, sh.sm, pad, eq_adj);
-      fin_store(sh.sm, ctl);
-      if (tid == 0) {
-        jc->pad = pad;
-        jc->eq_adj = eq_adj;
-      }
-    }
-    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  const uint32_t blk = blockIdx.x - 1;
   const JagSeg* seg = a.segs + (size_t)k * a.nseg;
   const uint32_t P = seg[a.nseg - 1].cum_pairs;
-  const T lam = ctl->lam, r = ctl->r;
   const T* E = a.erow + (((size_t)1 << (a.nrv - k - 1)) - 1);  // E_{nrv-k-1}
   const T* eint = a.eint + (((size_t)1 << a.niv) - 1);         // E_niv of the batch
   // Round k - 1's state: the input for k = 1, else state[(k - 1) & 1].
@@ -208,8 +189,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, 
   const T* src = from_in ? a.in : a.state[(k - 1) & 1];
   const size_t sstr = from_in ? a.in_stride : a.state_stride;
   T* dst = a.state[k & 1];
-  T acc[JG_NS] = {F::zero(), F::zero(), F::zero()};
-  for (uint32_t p = blk * PASS_THREADS + tid; p < P; p += nblk * PASS_THREADS) {
+  for (uint32_t p = start; p < P; p += stride) {
     uint32_t lo = 0, hi = a.nseg - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -246,6 +226,53 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, 
                                                          F::add(e[2], o[2]), F::add(e[3], o[3]))));
     acc[2] = F::add(acc[2], W);
   }
+}
+
+// Row round k as 1 + nblk blocks (see the file comment); block 0 warms the
+// hop on a scratch copy of the transcript state while the others stream.
+template <class F, bool FOLD>
+__global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, uint32_t k) {
+  using T = typename F::T;
+  __shared__ JagLds<F> L;
+  __shared__ HopLds warm;
+  auto& sh = L.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
+  JagCtl<F>* jc = a.ctl;
+  Ctl<F>* ctl = &jc->c;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_load(sh.sm, ctl);
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       4u * (uint32_t)sizeof(T), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) {
+      sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+      L.pad = jc->pad;
+      L.eq_adj = jc->eq_adj;
+    }
+    __syncthreads();
+    if (!sh.is_last) return;
+    slot_rows_sum<F, JG_NS>(a.slots, sh);
+    if (tid < 64) {
+      T pad = L.pad, eq_adj = L.eq_adj;
+      jag_finish<F, HOP_PASS>(a, k, sh.out, sh.sm, pad, eq_adj);
+      fin_store(sh.sm, ctl);
+      if (tid == 0) {
+        jc->pad = pad;
+        jc->eq_adj = eq_adj;
+      }
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  T acc[JG_NS] = {F::zero(), F::zero(), F::zero()};
+  jag_row_pairs<F, FOLD>(a, k, ctl->lam, ctl->r, blk * PASS_THREADS + tid, nblk * PASS_THREADS,
+                         acc);
   block_reduce<F, JG_NS, PASS_THREADS>(acc, sh.lds, sh.out);
   slot_row_add<F, JG_NS>(a.slots, blk, sh.out);
   __builtin_amdgcn_s_waitcnt(0);
@@ -253,11 +280,14 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, 
   if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The batch phase and the final fold, one block: the last row round's state
-// folded at its challenge into one element a segment, then each batch round
-// sums its pairs, finishes and folds.
+// One block: row rounds kfirst..nrv-1 (those whose pairs are too few to
+// fill the device: a launch and a device-wide arrival each would cost more
+// than the pairs), then the last row round's state folded at its challenge
+// into one element a segment, each batch round (sum its pairs, finish, fold)
+// and the final fold.
 template <class F>
-__global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a) {
+__global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a,
+                                                                    uint32_t kfirst) {
   using T = typename F::T;
   __shared__ JagLds<F> L;
   auto& sh = L.round;
@@ -271,6 +301,23 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a
   }
   __syncthreads();
   const T lam = ctl->lam;
+  for (uint32_t k = kfirst; k < a.nrv; ++k) {
+    T acc[JG_NS] = {F::zero(), F::zero(), F::zero()};
+    if (k == 0)
+      jag_row_pairs<F, false>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc);
+    else
+      jag_row_pairs<F, true>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc);
+    block_reduce<F, JG_NS, PASS_THREADS>(acc, sh.lds, sh.out);
+    if (tid < 64) {
+      T pad = L.pad, eq_adj = L.eq_adj;
+      jag_finish<F, HOP_INLINE>(a, k, sh.out, sh.sm, pad, eq_adj);
+      if (tid == 0) {
+        L.pad = pad;
+        L.eq_adj = eq_adj;
+      }
+    }
+    __syncthreads();
+  }
   // Boundary fold at r_{nrv-1}: segment s's rows 0, 1 of the last row round.
   {
     const uint32_t k = a.nrv - 1;
@@ -311,7 +358,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a
     block_reduce<F, JG_NS, PASS_THREADS>(acc, sh.lds, sh.out);
     if (tid < 64) {
       T pad = L.pad, eq_adj = L.eq_adj;
-      jag_finish<F>(a, a.nrv + m, sh.out, sh.sm, pad, eq_adj);
+      jag_finish<F, HOP_INLINE>(a, a.nrv + m, sh.out, sh.sm, pad, eq_adj);
       if (tid == 0) {
         L.pad = pad;
         L.eq_adj = eq_adj;
@@ -507,10 +554,13 @@ static int prove_jagged_typed(zs_ctx* ctx, const zs_jagged_args& a, zs_transcrip
   };
   uint64_t live_prev = width;
   const uint32_t maxg = ctx->tune.max_grid < (uint32_t)MAXG ? ctx->tune.max_grid : (uint32_t)MAXG;
+  uint32_t kfirst = 0;
   for (uint32_t k = 0; k < nrv; ++k) {
     const JagSeg& last = segs[(size_t)k * nseg + nseg - 1];
     const uint32_t P = last.cum_pairs;
     const uint64_t live = (uint64_t)last.base + last.count;
+    if (P <= JG_TAIL_PAIRS) break;  // this round and the rest: one block
+    kfirst = k + 1;
     size_t grid = (P + PASS_THREADS - 1) / PASS_THREADS;
     if (grid > maxg) grid = maxg;
     if (grid < 1) grid = 1;
@@ -526,9 +576,20 @@ static int prove_jagged_typed(zs_ctx* ctx, const zs_jagged_args& a, zs_transcrip
     live_prev = live;
   }
   hipEvent_t evb = begin();
-  jagged_batch_kernel<F><<<1, PASS_THREADS, 0, s>>>(g);
+  jagged_batch_kernel<F><<<1, PASS_THREADS, 0, s>>>(g, kfirst);
   ZS_HIP(hipGetLastError());
-  mark(evb, nrv, KREC_JAGGED_BATCH, 1, (uint64_t)4 * sizeof(T) * (live_prev + 2 * nseg));
+  // Row rounds read the previous round's rows and write their own (round 0
+  // only reads); the batch phase reads the last row state once.
+  uint64_t tail_rows = 2 * (uint64_t)nseg;
+  for (uint32_t k = kfirst; k < nrv; ++k) {
+    const JagSeg& last = segs[(size_t)k * nseg + nseg - 1];
+    const uint64_t live = (uint64_t)last.base + last.count;
+    tail_rows += k == 0 ? live : live_prev + live;
+    live_prev = live;
+  }
+  tail_rows += live_prev;
+  const uint64_t tail_bytes = (uint64_t)4 * sizeof(T) * tail_rows;
+  mark(evb, kfirst, KREC_JAGGED_BATCH, 1, tail_bytes);
   ZS_HIP(hipMemcpyAsync(h, b, out_bytes, hipMemcpyDeviceToHost, s));
   ZS_HIP(hipStreamSynchronize(s));
   const JagCtl<F>& jo = *(const JagCtl<F>*)(h + o_ctl);

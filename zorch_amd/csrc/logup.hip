@@ -16,6 +16,8 @@ namespace zs {
 
 constexpr int LG_TABLES = 5;  // [eq, n0, d1, n1, d0]
 constexpr int LG_NA = 4;      // message nodes t = 0..3 (degree 3)
+// Rounds with at most this many pairs run in one block (logup_tail_kernel).
+constexpr size_t LG_TAIL_PAIRS = 2048;
 
 // eq (lam (n0 d1 + n1 d0) + d0 d1) on v = [eq, n0, d1, n1, d0].
 template <class F>
@@ -25,53 +27,25 @@ __device__ __forceinline__ typename F::T logup_combine(typename F::T lam,
   return F::mul(v[0], F::add(F::mul(lam, cross), F::mul(v[4], v[2])));
 }
 
-// Round `round` of a LogUp stage as 1 + nblk blocks. FOLD: `src` holds the
-// previous round's tables (length 2L each), folded at its challenge into
-// `work`; else `src` holds this round's (round 0: the input).
+// Pairs start, start + stride, ... of a round of L entries a table: fold
+// (FOLD: `src` holds the previous round's tables, 2L entries each, folded at
+// r into `work`; pair j reads and writes only positions j, j + L/2 of this
+// round and j + L, j + 3L/2 of the previous, so the fold runs in place) and
+// add the summand at t = 0..3 to acc.
 template <class F, bool FOLD>
-__global__ __launch_bounds__(PASS_THREADS) void logup_pass_kernel(StageArgs<F, F> a,
-                                                                  const typename F::T* src,
-                                                                  size_t src_stride, uint32_t L,
-                                                                  uint32_t round,
-                                                                  uint32_t need_claim) {
+__device__ __forceinline__ void logup_pairs(const StageArgs<F, F>& a, const typename F::T* src,
+                                            size_t src_stride, uint32_t L, typename F::T lam,
+                                            typename F::T r, size_t start, size_t stride,
+                                            typename F::T (&acc)[LG_NA]) {
   using T = typename F::T;
-  __shared__ RoundLds<F, LG_NA> sh;
-  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
-  Ctl<F>* ctl = a.ctl;
-  if (blockIdx.x == 0) {
-    if (tid < 64) fin_load(sh.sm, ctl);
-    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
-    __syncthreads();
-    if (!sh.is_last) return;
-    slot_rows_sum<F, LG_NA>(a.slots, sh);
-    if (tid < 64) {
-      // The message shape of a degree-3 product round without the eq factor
-      // (the eq table is folded like the others).
-      finish_round<F, F, LG_NA - 1, false, HOP_PASS>(a, round, need_claim != 0, sh.out, sh.sm,
-                                                     a.work, a.in,
-                                                     round == 0 ? a.in_stride : a.work_stride,
-                                                     round == 0);
-      fin_store(sh.sm, ctl);
-    }
-    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  const uint32_t blk = blockIdx.x - 1;
-  const T lam = ctl->lam, r = ctl->r;
   const size_t half = L >> 1;
-  T acc[LG_NA];
-#pragma unroll
-  for (int t = 0; t < LG_NA; ++t) acc[t] = F::zero();
-  for (size_t j = (size_t)blk * PASS_THREADS + tid; j < half; j += (size_t)nblk * PASS_THREADS) {
+  for (size_t j = start; j < half; j += stride) {
     T lo[LG_TABLES], df[LG_TABLES];
 #pragma unroll
     for (int q = 0; q < LG_TABLES; ++q) {
       const T* s = src + (size_t)q * src_stride;
       T x0, x1;
       if constexpr (FOLD) {
-        // Pair j of this round: entries j and j + L/2, each the previous
-        // round's pair (i, i + L) folded at r. A thread reads and writes only
-        // its own pair's positions, so the fold runs in place.
         const T a0 = s[j], a1 = s[j + L], b0 = s[j + half], b1 = s[j + half + L];
         x0 = F::add(a0, F::mul(r, F::sub(a1, a0)));
         x1 = F::add(b0, F::mul(r, F::sub(b1, b0)));
@@ -94,11 +68,103 @@ __global__ __launch_bounds__(PASS_THREADS) void logup_pass_kernel(StageArgs<F, F
       }
     }
   }
+}
+
+// Round `round` of a LogUp stage as 1 + nblk blocks. Blocks 1..nblk stream
+// the pairs (logup_pairs), add their sums to the slot rows and arrive; block
+// 0 warms the hop on a scratch copy of the transcript state (the finisher's
+// code then sits in its compute unit's instruction cache), waits for the
+// arrivals and finishes the round (finish_round: the message, the
+// Fiat-Shamir hop, the challenge, the claim reduction and, after the last
+// round, the five folded values).
+template <class F, bool FOLD>
+__global__ __launch_bounds__(PASS_THREADS) void logup_pass_kernel(StageArgs<F, F> a,
+                                                                  const typename F::T* src,
+                                                                  size_t src_stride, uint32_t L,
+                                                                  uint32_t round,
+                                                                  uint32_t need_claim) {
+  using T = typename F::T;
+  __shared__ PassLds<F, LG_NA> ps;
+  RoundLds<F, LG_NA>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
+  Ctl<F>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_load(sh.sm, ctl);
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       (uint32_t)(LG_NA * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    slot_rows_sum<F, LG_NA>(a.slots, sh);
+    if (tid < 64) {
+      // The message shape of a degree-3 product round without the eq factor
+      // (the eq table is folded like the others).
+      finish_round<F, F, LG_NA - 1, false, HOP_PASS>(a, round, need_claim != 0, sh.out, sh.sm,
+                                                     a.work, a.in,
+                                                     round == 0 ? a.in_stride : a.work_stride,
+                                                     round == 0);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - 1;
+  T acc[LG_NA];
+#pragma unroll
+  for (int t = 0; t < LG_NA; ++t) acc[t] = F::zero();
+  logup_pairs<F, FOLD>(a, src, src_stride, L, ctl->lam, ctl->r,
+                       (size_t)blk * PASS_THREADS + tid, (size_t)nblk * PASS_THREADS, acc);
   block_reduce<F, LG_NA, PASS_THREADS>(acc, sh.lds, sh.out);
   slot_row_add<F, LG_NA>(a.slots, blk, sh.out);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Rounds first..last-1 of a LogUp stage in one block: each round streams its
+// pairs over the block, reduces in LDS and wave 0 finishes it; the next round
+// folds at the challenge the finisher left in LDS. For the rounds whose pairs
+// are too few to fill the device, where a launch per round and a device-wide
+// arrival cost more than the pairs.
+template <class F>
+__global__ __launch_bounds__(PASS_THREADS) void logup_tail_kernel(StageArgs<F, F> a,
+                                                                  uint32_t first, uint32_t last) {
+  using T = typename F::T;
+  __shared__ RoundLds<F, LG_NA> sh;
+  const uint32_t tid = threadIdx.x;
+  Ctl<F>* ctl = a.ctl;
+  if (tid < 64) fin_load(sh.sm, ctl);
+  __syncthreads();
+  const T lam = ctl->lam;
+  for (uint32_t round = first; round < last; ++round) {
+    const uint32_t L = 1u << (a.n - round);
+    T acc[LG_NA];
+#pragma unroll
+    for (int t = 0; t < LG_NA; ++t) acc[t] = F::zero();
+    const T r = sh.sm.r;
+    if (round == 0)
+      logup_pairs<F, false>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
+    else if (round == 1)
+      logup_pairs<F, true>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
+    else
+      logup_pairs<F, true>(a, a.work, a.work_stride, L, lam, r, tid, PASS_THREADS, acc);
+    // The folded entries reach the next round's readers through the barriers
+    // (one workgroup).
+    block_reduce<F, LG_NA, PASS_THREADS>(acc, sh.lds, sh.out);
+    if (tid < 64)
+      finish_round<F, F, LG_NA - 1, false, HOP_INLINE>(a, round, round + 1 == last, sh.out, sh.sm,
+                                                     a.work, a.in,
+                                                     round == 0 ? a.in_stride : a.work_stride,
+                                                     round == 0);
+    __syncthreads();
+  }
+  if (tid < 64) fin_store(sh.sm, ctl);
 }
 
 // A LogUp stage (zs_prove, zs_state_*): buffers laid out as the product
@@ -175,6 +241,23 @@ class LogupInstance final : public Instance {
     for (uint32_t r = first; r < last; ++r) {
       const uint32_t L = 1u << (n_ - r);
       const size_t pairs = L / 2;
+      if (pairs <= LG_TAIL_PAIRS) {
+        // Every remaining round in one block.
+        hipEvent_t ev = nullptr;
+        if (ctx_->profile) {
+          ev = ctx_->event();
+          (void)hipEventRecord(ev, s);
+        }
+        logup_tail_kernel<F><<<1, PASS_THREADS, 0, s>>>(args_, r, last);
+        ZS_HIP(hipGetLastError());
+        if (ev) {
+          hipEvent_t stop = ctx_->event();
+          (void)hipEventRecord(stop, s);
+          const uint64_t b = (uint64_t)LG_TABLES * 3 * L * sizeof(T);
+          ctx_->records.push_back(KernelRecord{r, KREC_TAIL, 1, b, ev, stop});
+        }
+        break;
+      }
       const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
       size_t grid = (pairs + per_block - 1) / per_block;
       const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
