@@ -2,7 +2,9 @@
 
 A batch mixes kinds (field pairs, product / zerocheck, factor counts) and
 sizes; every member must equal the C oracle on the same inputs and the
-one-proof-per-call path, bit for bit, whatever the launch shapes.
+one-proof-per-call path, bit for bit, whatever the launch shapes. Members
+whose round 2 is a pass round take rounds 0-2 from the input (two launches
+of their own in steps 0 and 1) unless the lookahead setting is off.
 """
 
 from __future__ import annotations
@@ -88,7 +90,11 @@ def test_mixed_batch_matches_c_oracle():
                                   dict(tail_pairs=1 << 20, max_grid=512, pairs_per_thread=1,
                                        step_rounds=1),
                                   dict(tail_pairs=1024, max_grid=512, pairs_per_thread=2,
-                                       step_rounds=3)])
+                                       step_rounds=3),
+                                  # every member's rounds one step at a time (no rounds
+                                  # 0-2 from the input)
+                                  dict(tail_pairs=64, max_grid=4096, pairs_per_thread=4,
+                                       lookahead=0)])
 def test_batch_independent_of_launch_shapes(tune):
     ctx = default_context()
     insts = _mixed()

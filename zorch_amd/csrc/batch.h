@@ -6,7 +6,10 @@
 // remaining rounds. The per-round latency chain (reduction, Fiat-Shamir hop,
 // launch) is then paid once per step for the whole batch instead of once per
 // round per proof, and the finishers of different members run on different
-// CUs at the same time.
+// CUs at the same time. A member whose round 2 is a pass round takes its
+// first three rounds from the input, as a single proof does (lookahead on):
+// step 0 runs its rounds 0 and 1 (batch01_kernel), step 1 its round 2
+// (batch2_kernel), and from step 2 on it is one round ahead of the step.
 //
 // Device block layout (one pool allocation):
 //   io records (one per member: control block | messages | raw point | final
@@ -27,7 +30,8 @@ template <class FT, class FC, int K, int NPTS, bool EQ>
 class BatchGroupImpl final : public BatchGroup {
   using TT = typename FT::T;
   using TC = typename FC::T;
-  static constexpr size_t SLOT_BYTES = (size_t)NSLOT * SLOTW * sizeof(uint64_t);
+  // Slot rows of the passes, then the look-ahead rows rounds 0-1 sum into.
+  static constexpr size_t SLOT_BYTES = ((size_t)NSLOT * SLOTW + LA_SLOT_WORDS) * sizeof(uint64_t);
 
   struct Member {
     size_t index = 0;
@@ -36,10 +40,17 @@ class BatchGroupImpl final : public BatchGroup {
     uint32_t n = 0;
     size_t rec = 0, fam_l = 0, fam_r = 0, work = 0, in_stage = 0;
   };
-  struct Step {
+  // One launch of a step: kind 0 = batch_kernel (passes of one round type and
+  // tails), 1 = batch01_kernel (rounds 0-1 from the input), 2 = batch2_kernel
+  // (round 2 from the input).
+  struct Launch {
+    uint32_t kind = 0;
     size_t item_off = 0;
     uint32_t nitems = 0;
     uint32_t blocks = 0;
+  };
+  struct Step {
+    std::vector<Launch> launches;
     uint64_t bytes = 0;  // algorithmic bytes of the step (fused-pass model)
   };
 
@@ -187,27 +198,35 @@ class BatchGroupImpl final : public BatchGroup {
   int enqueue_step(uint32_t st, hipStream_t s) override {
     if (st >= steps_.size()) return ZS_OK;
     const Step& S = steps_[st];
-    if (S.nitems == 0) return ZS_OK;
+    if (S.launches.empty()) return ZS_OK;
     uint8_t* b = (uint8_t*)block_;
     const StageArgs<FT, FC>* args = (const StageArgs<FT, FC>*)(b + args_off_);
-    const BatchItem* items = (const BatchItem*)(b + items_off_) + S.item_off;
     hipEvent_t ev = nullptr;
     if (ctx_->profile) {
       ev = ctx_->event();
       (void)hipEventRecord(ev, s);
     }
-    if (st == 0) {
-      batch_kernel<FT, FC, K, EQ, 0><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
-    } else if (st == 1) {
-      batch_kernel<FT, FC, K, EQ, 1><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
-    } else {
-      batch_kernel<FT, FC, K, EQ, 2><<<S.blocks, PASS_THREADS, 0, s>>>(args, items, S.nitems);
+    uint32_t blocks = 0;
+    for (const Launch& L : S.launches) {
+      const BatchItem* items = (const BatchItem*)(b + items_off_) + L.item_off;
+      blocks += L.blocks;
+      if (L.kind == 1) {
+        batch01_kernel<FT, FC, K, EQ><<<L.blocks, PASS_THREADS, 0, s>>>(args, items, L.nitems);
+      } else if (L.kind == 2) {
+        batch2_kernel<FT, FC, K, EQ><<<L.blocks, PASS_THREADS, 0, s>>>(args, items, L.nitems);
+      } else if (st == 0) {
+        batch_kernel<FT, FC, K, EQ, 0><<<L.blocks, PASS_THREADS, 0, s>>>(args, items, L.nitems);
+      } else if (st == 1) {
+        batch_kernel<FT, FC, K, EQ, 1><<<L.blocks, PASS_THREADS, 0, s>>>(args, items, L.nitems);
+      } else {
+        batch_kernel<FT, FC, K, EQ, 2><<<L.blocks, PASS_THREADS, 0, s>>>(args, items, L.nitems);
+      }
+      ZS_HIP(hipGetLastError());
     }
-    ZS_HIP(hipGetLastError());
     if (ev) {
       hipEvent_t stop = ctx_->event();
       (void)hipEventRecord(stop, s);
-      ctx_->records.push_back(KernelRecord{st, KREC_BATCH, S.blocks, S.bytes, ev, stop});
+      ctx_->records.push_back(KernelRecord{st, KREC_BATCH, blocks, S.bytes, ev, stop});
     }
     return ZS_OK;
   }
@@ -243,17 +262,46 @@ class BatchGroupImpl final : public BatchGroup {
  private:
   // Items of every step: the next round of each running member, or the next
   // run of at most step_rounds tail rounds (all of them when 0); tails first
-  // (they run longest), then passes by decreasing share of the grid. A pass
-  // item of step s is round s of its member.
+  // (they run longest), then passes by decreasing share of the grid. The
+  // passes of a plain launch all have the step's round type (round 0, round 1,
+  // or later), since a member taking its first rounds from the input runs
+  // rounds 0-1 in step 0 and round 2 in step 1 in launches of their own.
   void build_schedule() {
     std::vector<uint32_t> next(members_.size(), 0);
     const uint32_t maxg = tune_.max_grid < (uint32_t)MAXG ? tune_.max_grid : (uint32_t)MAXG;
+    const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
+    auto grid_for = [&](size_t work) {
+      size_t grid = (work + per_block - 1) / per_block;
+      if (grid > maxg) grid = maxg;
+      if (grid < 1) grid = 1;
+      while (grid & (grid - 1)) grid &= grid - 1;
+      return (uint32_t)grid;
+    };
+    std::vector<bool> fused(members_.size(), false);
+    for (size_t i = 0; i < members_.size(); ++i) {
+      const uint32_t n = members_[i].n;
+      fused[i] = tune_.lookahead && n >= 4 && ((size_t)1 << (n - 3)) > tune_.tail_pairs;
+    }
     for (uint32_t st = 0; st < max_n_; ++st) {
-      std::vector<BatchItem> tails, passes;
+      std::vector<BatchItem> tails, passes, first, second;
       Step S;
       for (size_t i = 0; i < members_.size(); ++i) {
         const uint32_t n = members_[i].n, round = next[i];
         if (round >= n) continue;
+        const size_t N = (size_t)1 << n;
+        if (fused[i] && round == 0) {
+          // Rounds 0-1 stream 16-byte vectors of groups, as a single proof does.
+          first.push_back(BatchItem{(uint32_t)i, grid_for(N / 4 / (16 / sizeof(TT))), 0, 0});
+          next[i] = 2;
+          S.bytes += (uint64_t)K * N * sizeof(TT);
+          continue;
+        }
+        if (fused[i] && round == 2) {
+          second.push_back(BatchItem{(uint32_t)i, grid_for(N / 8), 0, 2});
+          next[i] = 3;
+          S.bytes += (uint64_t)K * N * sizeof(TT) + (uint64_t)K * (N / 4) * sizeof(TC);
+          continue;
+        }
         const size_t pairs = (size_t)1 << (n - round - 1);
         if (pairs <= tune_.tail_pairs) {
           const uint32_t end =
@@ -263,28 +311,33 @@ class BatchGroupImpl final : public BatchGroup {
           for (uint32_t r = round; r < end; ++r) S.bytes += pass_bytes(n, r);
           continue;
         }
-        const size_t per_block = (size_t)PASS_THREADS * tune_.pairs_per_thread;
-        size_t grid = (pairs + per_block - 1) / per_block;
-        if (grid > maxg) grid = maxg;
-        if (grid < 1) grid = 1;
-        while (grid & (grid - 1)) grid &= grid - 1;
-        passes.push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, st});
-        next[i] = st + 1;
-        S.bytes += pass_bytes(n, st);
+        passes.push_back(BatchItem{(uint32_t)i, grid_for(pairs), 0, round});
+        next[i] = round + 1;
+        S.bytes += pass_bytes(n, round);
       }
-      std::stable_sort(passes.begin(), passes.end(),
-                       [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; });
-      S.item_off = items_.size();
-      uint32_t off = 0;
-      for (auto* list : {&tails, &passes}) {
-        for (BatchItem it : *list) {
-          it.block_off = off;
-          off += it.blocks;
-          items_.push_back(it);
+      auto by_share = [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; };
+      std::stable_sort(passes.begin(), passes.end(), by_share);
+      std::stable_sort(first.begin(), first.end(), by_share);
+      std::stable_sort(second.begin(), second.end(), by_share);
+      auto add_launch = [&](uint32_t kind, std::initializer_list<std::vector<BatchItem>*> lists) {
+        Launch L;
+        L.kind = kind;
+        L.item_off = items_.size();
+        uint32_t off = 0;
+        for (auto* list : lists) {
+          for (BatchItem it : *list) {
+            it.block_off = off;
+            off += it.blocks;
+            items_.push_back(it);
+          }
         }
-      }
-      S.nitems = (uint32_t)(items_.size() - S.item_off);
-      S.blocks = off;
+        L.nitems = (uint32_t)(items_.size() - L.item_off);
+        L.blocks = off;
+        if (L.nitems) S.launches.push_back(L);
+      };
+      add_launch(0, {&tails, &passes});
+      add_launch(1, {&first});
+      add_launch(2, {&second});
       steps_.push_back(S);
     }
   }

@@ -2862,84 +2862,80 @@ struct Fused01Lds {
   HopLds warm;
 };
 
-// Rounds 0 and 1 of one proof as 1 + nblk blocks (round 1 < n - 1).
+// The finisher of rounds 0 and 1 (one block, after every streaming block of
+// the proof has added its coefficient sums): round 0's message from round
+// 1's coefficient sums, its hop, round 1's node sums at r_0, its hop. The
+// proof state must already be in sh.sm (fin_load, fin_eq of round 0).
+template <class FT, class FC, int K, bool EQ, int HOP>
+__device__ __forceinline__ void fused01_finish(const StageArgs<FT, FC>& a,
+                                               RoundLds<FC, (K + 1) * (K + 1)>& sh) {
+  using T = typename FC::T;
+  constexpr int NA = K + 1, NC = NA * NA;
+  constexpr int W = SlotOps<FC>::W;
+  const uint32_t tid = threadIdx.x;
+  Ctl<FC>* ctl = a.ctl;
+  StageArgs<FT, FC> af = a;
+  af.final_out = nullptr;
+  static_assert(sizeof(sh.sum64) == (PASS_THREADS / LA_SLOTW) * LA_SLOTW * sizeof(uint64_t),
+                "slot scratch shapes");
+  la_slot_sum<FC, NC>(la_rows(a.slots, 1), NA * W,
+                      reinterpret_cast<uint64_t (*)[LA_SLOTW]>(&sh.sum64[0][0]), sh.out);
+  // Round 0's node sums from round 1's coefficients (fused01_stream):
+  // node t = sum_i t^i (e0 c[0][i] + e1 c[1][i]), e_b = eq(w_1, b).
+  if (tid < (uint32_t)NA) {
+    T e0 = FC::one(), e1 = FC::one();
+    if constexpr (EQ) {
+      e0 = a.eq_ab[3];                   // 1 - w_1
+      e1 = FC::add(a.eq_ab[2], a.eq_ab[3]);  // w_1
+    }
+    T v = FC::zero();
+#pragma unroll
+    for (int i = K; i >= 0; --i) {
+      const T u = FC::add(FC::mul(e0, sh.out[i]), FC::mul(e1, sh.out[NA + i]));
+      T vt = FC::zero();
+      for (uint32_t j = 0; j < tid; ++j) vt = FC::add(vt, v);  // v * t, t <= K
+      v = FC::add(vt, u);
+    }
+    sh.lds[tid] = v;
+  }
+  __syncthreads();
+  if (tid == 0) trace_mark(a.trace, 0, 2);
+  if (tid < 64)
+    finish_round<FT, FC, K, EQ, HOP>(af, 0, false, sh.lds, sh.sm, a.work, a.in,
+                                           a.in_stride, true);
+  // Round 1's node sums: its coefficients at r_0.
+  if (tid < (uint32_t)NA) {
+    const T r = sh.sm.r;
+    T v = sh.out[tid * NA + K];
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) v = FC::add(sh.out[tid * NA + i], FC::mul(r, v));
+    sh.lds[tid] = v;
+  }
+  __syncthreads();
+  if (tid < (uint32_t)NA) sh.out[tid] = sh.lds[tid];
+  if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, 1);
+  __syncthreads();
+  if (tid == 0) trace_mark(a.trace, 1, 2);
+  if (tid < 64) {
+    finish_round<FT, FC, K, EQ, HOP>(af, 1, false, sh.out, sh.sm, a.work, a.in,
+                                           a.in_stride, true);
+    fin_store(sh.sm, ctl);
+  }
+}
+
+// One streaming block of rounds 0 and 1: block `blk` of `nblk` sums its
+// groups' (K+1)^2 coefficients and adds them to the look-ahead rows of round
+// 1 (the caller then arrives).
 template <class FT, class FC, int K, bool EQ>
-__global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kernel(
-    StageArgs<FT, FC> a) {
+__device__ __forceinline__ void fused01_stream_block(const StageArgs<FT, FC>& a, uint32_t blk,
+                                                     uint32_t nblk,
+                                                     RoundLds<FC, (K + 1) * (K + 1)>& sh) {
   using T = typename FC::T;
   using S = typename FT::T;
   constexpr int NA = K + 1, NC = NA * NA;
-  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
   constexpr int W = SlotOps<FC>::W;
-  __shared__ Fused01Lds<FC, K> ps;
-  RoundLds<FC, NC>& sh = ps.round;
-  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1, n = a.n;
+  const uint32_t tid = threadIdx.x, n = a.n;
   Ctl<FC>* ctl = a.ctl;
-  if (blockIdx.x == 0) {
-    if (tid < 64) {
-      fin_eq<FT, FC, EQ>(sh.sm, a, 0);
-      fin_load(sh.sm, ctl);
-      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
-      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
-      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
-      wave_sync();
-      fs_hop<HOP_FUSED>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
-                        (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
-    }
-    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
-    __syncthreads();
-    if (!sh.is_last) return;
-    StageArgs<FT, FC> af = a;
-    af.final_out = nullptr;
-    static_assert(sizeof(sh.sum64) == (PASS_THREADS / LA_SLOTW) * LA_SLOTW * sizeof(uint64_t),
-                  "slot scratch shapes");
-    la_slot_sum<FC, NC>(la_rows(a.slots, 1), NA * W,
-                        reinterpret_cast<uint64_t (*)[LA_SLOTW]>(&sh.sum64[0][0]), sh.out);
-    // Round 0's node sums from round 1's coefficients (fused01_stream):
-    // node t = sum_i t^i (e0 c[0][i] + e1 c[1][i]), e_b = eq(w_1, b).
-    if (tid < (uint32_t)NA) {
-      T e0 = FC::one(), e1 = FC::one();
-      if constexpr (EQ) {
-        e0 = a.eq_ab[3];                   // 1 - w_1
-        e1 = FC::add(a.eq_ab[2], a.eq_ab[3]);  // w_1
-      }
-      T v = FC::zero();
-#pragma unroll
-      for (int i = K; i >= 0; --i) {
-        const T u = FC::add(FC::mul(e0, sh.out[i]), FC::mul(e1, sh.out[NA + i]));
-        T vt = FC::zero();
-        for (uint32_t j = 0; j < tid; ++j) vt = FC::add(vt, v);  // v * t, t <= K
-        v = FC::add(vt, u);
-      }
-      sh.lds[tid] = v;
-    }
-    __syncthreads();
-    if (tid == 0) trace_mark(a.trace, 0, 2);
-    if (tid < 64)
-      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 0, false, sh.lds, sh.sm, a.work, a.in,
-                                             a.in_stride, true);
-    // Round 1's node sums: its coefficients at r_0.
-    if (tid < (uint32_t)NA) {
-      const T r = sh.sm.r;
-      T v = sh.out[tid * NA + K];
-#pragma unroll
-      for (int i = K - 1; i >= 0; --i) v = FC::add(sh.out[tid * NA + i], FC::mul(r, v));
-      sh.lds[tid] = v;
-    }
-    __syncthreads();
-    if (tid < (uint32_t)NA) sh.out[tid] = sh.lds[tid];
-    if (tid < 64) fin_eq<FT, FC, EQ>(sh.sm, a, 1);
-    __syncthreads();
-    if (tid == 0) trace_mark(a.trace, 1, 2);
-    if (tid < 64) {
-      finish_round<FT, FC, K, EQ, HOP_FUSED>(af, 1, false, sh.out, sh.sm, a.work, a.in,
-                                             a.in_stride, true);
-      fin_store(sh.sm, ctl);
-    }
-    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  const uint32_t blk = blockIdx.x - 1;
   const uint32_t q4 = 1u << (n - 2);  // round 1's pairs
   const uint32_t start = blk * PASS_THREADS + tid, stride = nblk * PASS_THREADS;
   const T* wl1 = nullptr;
@@ -2985,52 +2981,56 @@ __global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kerne
   }
   block_reduce<FC, NC, PASS_THREADS>(c, sh.lds, sh.out);
   la_slot_add<FC, NC>(la_rows(a.slots, 1), blk, NA * W, sh.out);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Round 2 of one proof from the input as 1 + nblk blocks: pair j's table
-// entries T2[j + h N/8] (h = 0, 1) are the input entries at j + h N/8 +
-// {0, N/2, N/4, 3N/4} folded at r_0 and r_1 (Fold2); they are written to
-// `work` and evaluated; block 0 finishes round 2 as in pass_kernel. r_0 is
-// read from the raw point (the control block holds r_1).
+// Rounds 0 and 1 of one proof as 1 + nblk blocks (round 1 < n - 1).
 template <class FT, class FC, int K, bool EQ>
-__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
-    StageArgs<FT, FC> a, uint32_t need_claim) {
+__global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void fused01_kernel(
+    StageArgs<FT, FC> a) {
   using T = typename FC::T;
   using S = typename FT::T;
-  constexpr int NA = K + 1;
+  constexpr int NA = K + 1, NC = NA * NA;
   constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
-  __shared__ PassLds<FC, NA> ps;
-  RoundLds<FC, NA>& sh = ps.round;
+  constexpr int W = SlotOps<FC>::W;
+  __shared__ Fused01Lds<FC, K> ps;
+  RoundLds<FC, NC>& sh = ps.round;
   const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1, n = a.n;
   Ctl<FC>* ctl = a.ctl;
   if (blockIdx.x == 0) {
     if (tid < 64) {
-      fin_eq<FT, FC, EQ>(sh.sm, a, 2);
+      fin_eq<FT, FC, EQ>(sh.sm, a, 0);
       fin_load(sh.sm, ctl);
       const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
       uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
       for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
       wave_sync();
-      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
-                       (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+      fs_hop<HOP_FUSED>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                        (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
     }
     if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
     __syncthreads();
     if (!sh.is_last) return;
-    if (tid == 0) trace_mark(a.trace, 2, 2);
-    slot_rows_sum<FC, NA>(a.slots, sh);
-    if (tid < 64) {
-      finish_round<FT, FC, K, EQ, HOP_PASS>(a, 2, need_claim != 0, sh.out, sh.sm, a.work, a.in,
-                                            a.work_stride, false);
-      fin_store(sh.sm, ctl);
-    }
+    fused01_finish<FT, FC, K, EQ, HOP_FUSED>(a, sh);
     if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  const uint32_t blk = blockIdx.x - 1;
+  fused01_stream_block<FT, FC, K, EQ>(a, blockIdx.x - 1, nblk, sh);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One streaming block of round 2 from the input: block `blk` of `nblk`
+// folds its pairs' input entries at r_0 and r_1, writes them to `work`, and
+// adds its node sums to the slot rows (the caller then arrives).
+template <class FT, class FC, int K, bool EQ>
+__device__ __forceinline__ void fold2_stream_block(const StageArgs<FT, FC>& a, uint32_t blk,
+                                                   uint32_t nblk, RoundLds<FC, K + 1>& sh) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1;
+  const uint32_t tid = threadIdx.x, n = a.n;
+  Ctl<FC>* ctl = a.ctl;
   const size_t q8 = (size_t)1 << (n - 3);  // round 2's pairs
   const size_t start = (size_t)blk * PASS_THREADS + tid, stride = (size_t)nblk * PASS_THREADS;
   // Weights of the two folds.
@@ -3121,6 +3121,49 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
   }
   block_reduce<FC, NA, PASS_THREADS>(acc, sh.lds, sh.out);
   slot_row_add<FC, NA>(a.slots, blk, sh.out);
+}
+
+// Round 2 of one proof from the input as 1 + nblk blocks: pair j's table
+// entries T2[j + h N/8] (h = 0, 1) are the input entries at j + h N/8 +
+// {0, N/2, N/4, 3N/4} folded at r_0 and r_1 (Fold2); they are written to
+// `work` and evaluated; block 0 finishes round 2 as in pass_kernel. r_0 is
+// read from the raw point (the control block holds r_1).
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void fold2_kernel(
+    StageArgs<FT, FC> a, uint32_t need_claim) {
+  using T = typename FC::T;
+  using S = typename FT::T;
+  constexpr int NA = K + 1;
+  constexpr int NPTS = K + 1 + (EQ ? 1 : 0);
+  __shared__ PassLds<FC, NA> ps;
+  RoundLds<FC, NA>& sh = ps.round;
+  const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1, n = a.n;
+  Ctl<FC>* ctl = a.ctl;
+  if (blockIdx.x == 0) {
+    if (tid < 64) {
+      fin_eq<FT, FC, EQ>(sh.sm, a, 2);
+      fin_load(sh.sm, ctl);
+      const uint32_t* from = reinterpret_cast<const uint32_t*>(static_cast<HopLds*>(&sh.sm));
+      uint32_t* to = reinterpret_cast<uint32_t*>(&ps.warm);
+      for (uint32_t i = tid; i < sizeof(HopLds) / 4; i += 64u) to[i] = from[i];
+      wave_sync();
+      fs_hop<HOP_PASS>((ZS_LDS HopLds*)&ps.warm, (const ZS_LDS uint8_t*)sh.sm.msg,
+                       (uint32_t)(NPTS * sizeof(T)), ctl->item_bytes, ctl->limbs, ctl->tr_kind);
+    }
+    if (tid == 0) sh.is_last = chain_wait(&ctl->counter, nblk, &ctl->fault) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.is_last) return;
+    if (tid == 0) trace_mark(a.trace, 2, 2);
+    slot_rows_sum<FC, NA>(a.slots, sh);
+    if (tid < 64) {
+      finish_round<FT, FC, K, EQ, HOP_PASS>(a, 2, need_claim != 0, sh.out, sh.sm, a.work, a.in,
+                                            a.work_stride, false);
+      fin_store(sh.sm, ctl);
+    }
+    if (tid == 0) __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  fold2_stream_block<FT, FC, K, EQ>(a, blockIdx.x - 1, nblk, sh);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0) __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -3205,6 +3248,88 @@ __global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch_kernel(
     round_block<FT, FC, FC, K, EQ, true>(a, a.work, a.work_stride, L, round, need_claim, blk,
                                          it.blocks, sh.round);
   }
+}
+
+// Rounds 0 and 1 of one batch member from the input as `nblk` blocks of a
+// step (fused01_kernel's work): every block streams its groups and adds its
+// coefficient sums; the last block to take a ticket finishes both rounds.
+template <class FT, class FC, int K, bool EQ>
+__device__ __forceinline__ void fused01_block(const StageArgs<FT, FC>& a, uint32_t blk,
+                                              uint32_t nblk,
+                                              RoundLds<FC, (K + 1) * (K + 1)>& sh) {
+  fused01_stream_block<FT, FC, K, EQ>(a, blk, nblk, sh);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t prev =
+        __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh.is_last = (prev == nblk - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!sh.is_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x < 64) {
+    fin_eq<FT, FC, EQ>(sh.sm, a, 0);
+    fin_load(sh.sm, a.ctl);
+  }
+  __syncthreads();
+  fused01_finish<FT, FC, K, EQ, HOP_PASS>(a, sh);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&a.ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Round 2 of one batch member from the input as `nblk` blocks of a step
+// (fold2_kernel's work); the last block to take a ticket finishes round 2.
+template <class FT, class FC, int K, bool EQ>
+__device__ __forceinline__ void fold2_block(const StageArgs<FT, FC>& a, bool need_claim,
+                                            uint32_t blk, uint32_t nblk,
+                                            RoundLds<FC, K + 1>& sh) {
+  fold2_stream_block<FT, FC, K, EQ>(a, blk, nblk, sh);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t prev =
+        __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh.is_last = (prev == nblk - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!sh.is_last) return;
+  if (threadIdx.x < 64) {
+    fin_eq<FT, FC, EQ>(sh.sm, a, 2);
+    fin_load(sh.sm, a.ctl);
+  }
+  slot_rows_sum<FC, K + 1>(a.slots, sh);
+  if (threadIdx.x < 64) {
+    finish_round<FT, FC, K, EQ, HOP_PASS>(a, 2, need_claim, sh.out, sh.sm, a.work, a.in,
+                                          a.work_stride, false);
+    fin_store(sh.sm, a.ctl);
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&a.ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Batch steps of members whose first rounds come from the input: rounds 0-1
+// (FIRST = 0) or round 2 (FIRST = 2) of every item (no tail items).
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) FUSED_WAVES_PER_EU void batch01_kernel(
+    const StageArgs<FT, FC>* __restrict__ args, const BatchItem* __restrict__ items,
+    uint32_t nitems) {
+  __shared__ RoundLds<FC, (K + 1) * (K + 1)> sh;
+  const uint32_t idx = find_item(items, nitems);
+  const BatchItem it = items[idx];
+  const StageArgs<FT, FC> a = args[it.inst];
+  fused01_block<FT, FC, K, EQ>(a, blockIdx.x - it.block_off, it.blocks, sh);
+}
+
+template <class FT, class FC, int K, bool EQ>
+__global__ __launch_bounds__(PASS_THREADS) PASS_WAVES_PER_EU void batch2_kernel(
+    const StageArgs<FT, FC>* __restrict__ args, const BatchItem* __restrict__ items,
+    uint32_t nitems) {
+  __shared__ RoundLds<FC, K + 1> sh;
+  const uint32_t idx = find_item(items, nitems);
+  const BatchItem it = items[idx];
+  const StageArgs<FT, FC> a = args[it.inst];
+  fold2_block<FT, FC, K, EQ>(a, 3 == a.n, blockIdx.x - it.block_off, it.blocks, sh);
 }
 
 // Suffix eq families of the two halves of the eq point (compute_eq_evaluations,
