@@ -6,7 +6,9 @@
 // batch_kernel with the round type chosen per item, so requests that arrive
 // while others are mid-proof join at the next step. A request whose round
 // has at most tail_pairs pairs runs all its remaining rounds in one block of
-// that step and completes with it.
+// that step and completes with it. A request whose round 2 is a pass round
+// takes its first three rounds from the input (lookahead on): rounds 0-1 in
+// one step (batch01_kernel), round 2 in the next (batch2_kernel).
 //
 // Request device block: io record (control block | messages | raw point |
 // final values | eq point | eq line coefficients) | slot rows | eq_l, eq_r
@@ -23,9 +25,11 @@ template <class FT, class FC, int K, int NPTS, bool EQ>
 class ServeKindImpl final : public ServeKind {
   using TT = typename FT::T;
   using TC = typename FC::T;
-  static constexpr size_t SLOT_BYTES = (size_t)NSLOT * SLOTW * sizeof(uint64_t);
+  // Slot rows of the passes, then the look-ahead rows rounds 0-1 sum into.
+  static constexpr size_t SLOT_BYTES = ((size_t)NSLOT * SLOTW + LA_SLOT_WORDS) * sizeof(uint64_t);
 
   struct Req final : ServeReq {
+    bool fused = false;  // rounds 0-2 from the input
     void* block = nullptr;
     void* host = nullptr;  // pinned mirror of the io record
     size_t rec_bytes = 0, out_bytes = 0;
@@ -99,6 +103,7 @@ class ServeKindImpl final : public ServeKind {
     r->n = n;
     r->next = 0;
     r->bytes = bytes;
+    r->fused = ctx->tune.lookahead && n >= 4 && ((size_t)1 << (n - 3)) > ctx->tune.tail_pairs;
     uint8_t* b = (uint8_t*)r->block;
     uint8_t* h = (uint8_t*)r->host;
     memset(h, 0, r->rec_bytes);
@@ -151,12 +156,27 @@ class ServeKindImpl final : public ServeKind {
     // Pass items by round type (round 0, round 1, later rounds), one launch
     // each: a launch's kernel is specialised to its round type. Tail items
     // ride with the later rounds.
-    std::vector<BatchItem> tails, passes[3];
+    std::vector<BatchItem> tails, passes[3], first01, second2;
+    const size_t per_block = (size_t)PASS_THREADS * tune.pairs_per_thread;
+    auto grid_for = [&](size_t work) {
+      size_t grid = (work + per_block - 1) / per_block;
+      if (grid > maxg) grid = maxg;
+      if (grid < 1) grid = 1;
+      while (grid & (grid - 1)) grid &= grid - 1;
+      return (uint32_t)grid;
+    };
     for (size_t i = 0; i < m; ++i) {
-      ServeReq* r = reqs[i];
+      Req* r = static_cast<Req*>(reqs[i]);
       const uint32_t round = r->next;
       const size_t pairs = (size_t)1 << (r->n - round - 1);
-      if (pairs <= tune.tail_pairs) {
+      const size_t N = (size_t)1 << r->n;
+      if (r->fused && round == 0) {
+        first01.push_back(BatchItem{(uint32_t)i, grid_for(N / 4 / (16 / sizeof(TT))), 0, 0});
+        r->next = 2;
+      } else if (r->fused && round == 2) {
+        second2.push_back(BatchItem{(uint32_t)i, grid_for(N / 8), 0, 2});
+        r->next = 3;
+      } else if (pairs <= tune.tail_pairs) {
         // A run of at most step_rounds tail rounds (all of them when 0).
         const uint32_t end = tune.step_rounds && r->n - round > tune.step_rounds
                                  ? round + tune.step_rounds
@@ -164,21 +184,16 @@ class ServeKindImpl final : public ServeKind {
         tails.push_back(BatchItem{(uint32_t)i, 1, 0, tail_item(round, end)});
         r->next = end;
       } else {
-        const size_t per_block = (size_t)PASS_THREADS * tune.pairs_per_thread;
-        size_t grid = (pairs + per_block - 1) / per_block;
-        if (grid > maxg) grid = maxg;
-        if (grid < 1) grid = 1;
-        while (grid & (grid - 1)) grid &= grid - 1;
-        passes[round < 2 ? round : 2].push_back(BatchItem{(uint32_t)i, (uint32_t)grid, 0, round});
+        passes[round < 2 ? round : 2].push_back(BatchItem{(uint32_t)i, grid_for(pairs), 0, round});
         r->next = round + 1;
       }
       r->finishing = r->next == r->n;
     }
-    for (auto& list : passes)
-      std::stable_sort(list.begin(), list.end(),
+    for (auto* list : {&passes[0], &passes[1], &passes[2], &first01, &second2})
+      std::stable_sort(list->begin(), list->end(),
                        [](const BatchItem& x, const BatchItem& y) { return x.blocks > y.blocks; });
     const size_t args_bytes = align256(m * sizeof(StageArgs<FT, FC>));
-    const size_t need = args_bytes + m * sizeof(BatchItem);
+    const size_t need = args_bytes + m * sizeof(BatchItem);  // one item a request
     // The next staging slot of the ring, free once the step that last used
     // it has run (steps are queued ahead of the device).
     Stage& st = ring_[ring_next_];
@@ -201,13 +216,16 @@ class ServeKindImpl final : public ServeKind {
     StageArgs<FT, FC>* args = (StageArgs<FT, FC>*)h;
     for (size_t i = 0; i < m; ++i) args[i] = static_cast<Req*>(reqs[i])->sa;
     BatchItem* items = (BatchItem*)(h + args_bytes);
-    // Launch t takes items [first[t], first[t] + count[t]) over blocks[t] blocks.
-    uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0}, blocks[3] = {0, 0, 0};
+    // Launch t takes items [first[t], first[t] + count[t]) over blocks[t] blocks:
+    // round 0, round 1, later rounds and tails, rounds 0-1 from the input,
+    // round 2 from the input.
+    uint32_t first[5] = {}, count[5] = {}, blocks[5] = {};
     uint32_t nitems = 0;
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < 5; ++t) {
       first[t] = nitems;
       uint32_t off = 0;
-      std::vector<BatchItem>* lists[2] = {&passes[t], t == 2 ? &tails : nullptr};
+      std::vector<BatchItem>* lists[2] = {t == 3 ? &first01 : t == 4 ? &second2 : &passes[t],
+                                          t == 2 ? &tails : nullptr};
       for (auto* list : lists) {
         if (!list) continue;
         for (BatchItem it : *list) {
@@ -232,6 +250,12 @@ class ServeKindImpl final : public ServeKind {
     if (count[2])
       batch_kernel<FT, FC, K, EQ, 2><<<blocks[2], PASS_THREADS, 0, s>>>(dargs, ditems + first[2],
                                                                         count[2]);
+    if (count[3])
+      batch01_kernel<FT, FC, K, EQ><<<blocks[3], PASS_THREADS, 0, s>>>(dargs, ditems + first[3],
+                                                                       count[3]);
+    if (count[4])
+      batch2_kernel<FT, FC, K, EQ><<<blocks[4], PASS_THREADS, 0, s>>>(dargs, ditems + first[4],
+                                                                      count[4]);
     ZS_HIP(hipGetLastError());
     ZS_HIP(hipEventRecord(st.ev, s));
     return ZS_OK;
