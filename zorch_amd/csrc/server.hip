@@ -84,7 +84,11 @@ struct Direct {
   bool eq = false;
 };
 constexpr uint32_t DIRECT_MIN_ROUNDS = 20;
-constexpr int LANES = 3;
+// Streams of the direct lanes. A process gets four hardware queues (HIP's
+// default); the server's step stream and two lanes leave one for the
+// caller's own stream: two streams sharing a queue run one after the other,
+// which with three lanes and a caller's stream throttled the served steps.
+constexpr int LANES = 2;
 constexpr size_t TR_BYTES = sizeof(zs_transcript) - offsetof(zs_transcript, sha);
 
 }  // namespace zs
