@@ -13,7 +13,7 @@ import pytest
 from oracle import fields as OF
 from oracle import jagged as OJ
 from zorch_amd.challenge import ChallengePolicy
-from zorch_amd.fields import as_field, from_storage
+from zorch_amd.fields import FieldArray, as_field, from_storage
 from zorch_amd.logup_gkr import (JaggedGkrLayer, RoundWidthCaps, build_jagged_pyramid,
                                  extract_jagged_outputs, jagged_layer_transition,
                                  prove_jagged_layer)
@@ -107,3 +107,25 @@ def test_prover_rejects_before_device_work():
                      (RoundWidthCaps(8, 8, 4), "plane width")]:
         with pytest.raises(ValueError, match=msg):
             prove_jagged_layer(dl, one, one, z5, None, challenges=ch, caps=bad)
+
+
+def test_mixed_field_first_layer_promotes():
+    """Base-field numerators under extension denominators: the transition
+    promotes and equals the all-extension layer's (reference
+    jagged_circuit_test.py:222-259)."""
+    import numpy as np
+    from zorch_amd.fields import babybear_mont
+    rng = np.random.default_rng(8)
+    counts = (3, 1, 5, 2)
+    w = sum(counts)
+    nb = [FieldArray(rng.integers(0, OF.BABYBEAR.p, size=(w, 1), dtype=np.uint64)
+                     .astype(np.uint32), babybear_mont) for _ in range(2)]
+    db = [FieldArray(rng.integers(0, OF.BABYBEAR.p, size=(w, 4), dtype=np.uint64)
+                     .astype(np.uint32), as_field(FIELD)) for _ in range(2)]
+    mixed = JaggedGkrLayer(nb[0], nb[1], db[0], db[1], counts)
+    ext = JaggedGkrLayer(nb[0].astype(FIELD), nb[1].astype(FIELD), db[0], db[1], counts)
+    a = jagged_layer_transition(mixed, (2, 1, 3, 1))
+    b = jagged_layer_transition(ext, (2, 1, 3, 1))
+    for x, y in zip((a.numerator_0, a.numerator_1, a.denominator_0, a.denominator_1),
+                    (b.numerator_0, b.numerator_1, b.denominator_0, b.denominator_1)):
+        assert x.field is y.field and _words(x) == _words(y)

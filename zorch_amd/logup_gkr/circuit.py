@@ -54,8 +54,17 @@ class LogUpGkrOutput:
     denominator: FieldArray
 
 
+def _common(*arrays: FieldArray):
+    """The arrays in one field: base-field ones lifted to the extension when
+    any is an extension (the reference promotes a first layer of base-field
+    numerators under extension denominators, circuit.py:360-372)."""
+    f = max((a.field for a in arrays), key=lambda x: x.degree)
+    return tuple(a if a.field is f else a.astype(f) for a in arrays)
+
+
 def _fold_pairs(n0, n1, d0, d1):
     """Even nodes become the next layer's 0-child, odd nodes its 1-child."""
+    n0, n1, d0, d1 = _common(n0, n1, d0, d1)
     n0e, n0o = n0[0::2], n0[1::2]
     n1e, n1o = n1[0::2], n1[1::2]
     d0e, d0o = d0[0::2], d0[1::2]
@@ -80,6 +89,7 @@ def build_pyramid(first: GkrLayer) -> list[GkrLayer]:
 
 def _interleave(child_0: FieldArray, child_1: FieldArray) -> FieldArray:
     """[c0[0], c1[0], c0[1], c1[1], ...]."""
+    child_0, child_1 = _common(child_0, child_1)
     both = stack([child_0, child_1], axis=1)
     return both.reshape(2 * child_0.shape[0])
 
@@ -178,17 +188,19 @@ def jagged_layer_transition(layer: JaggedGkrLayer, out_row_counts, out_width=Non
                              f"(needs at least {(rc + 1) // 2})")
     even, odd = _transition_gathers(layer.row_counts, out, out_width)
     live = layer.live
-    f = layer.numerator_0.field
+    n0, n1, d0, d1 = _common(layer.numerator_0, layer.numerator_1, layer.denominator_0,
+                             layer.denominator_1)
+    f = n0.field
 
     def gather(a: FieldArray, neutral: FieldArray):
         ext = concatenate([a[:live], neutral, zeros((1,), f)])
         return ext[even], ext[odd]
 
     z1, o1 = zeros((1,), f), ones((1,), f)
-    n0e, n0o = gather(layer.numerator_0, z1)
-    n1e, n1o = gather(layer.numerator_1, z1)
-    d0e, d0o = gather(layer.denominator_0, o1)
-    d1e, d1o = gather(layer.denominator_1, o1)
+    n0e, n0o = gather(n0, z1)
+    n1e, n1o = gather(n1, z1)
+    d0e, d0o = gather(d0, o1)
+    d1e, d1o = gather(d1, o1)
     return JaggedGkrLayer(n0e * d1e + n1e * d0e, n0o * d1o + n1o * d0o, d0e * d1e, d0o * d1o,
                           out)
 
