@@ -30,6 +30,9 @@ namespace zs {
 constexpr int JG_NS = 3;  // sums a round: ez, eh, es
 // Row rounds with at most this many pairs run in the one-block kernel.
 constexpr uint32_t JG_TAIL_PAIRS = 2048;
+// Layers of at most this many segments search a round's cumulative pair
+// counts in LDS (staged once per block and round) instead of global memory.
+constexpr uint32_t JG_LDS_SEGS = 4096;
 
 // Row round k, segment s.
 struct JagSeg {
@@ -174,14 +177,20 @@ struct JagLds {
 
 // Pairs p = start, start + stride, ... of row round k: fold (FOLD: round
 // k-1's rows at r into round k's state) and add W B(even), W B(even + odd)
-// and W to acc.
+// and W to acc. Every thread of the block calls it (it stages the round's
+// cumulative pair counts in `cum`, LDS of JG_LDS_SEGS words, when they fit).
 template <class F, bool FOLD>
 __device__ __forceinline__ void jag_row_pairs(const JagArgs<F>& a, uint32_t k, typename F::T lam,
                                               typename F::T r, uint32_t start, uint32_t stride,
-                                              typename F::T (&acc)[JG_NS]) {
+                                              typename F::T (&acc)[JG_NS], uint32_t* cum) {
   using T = typename F::T;
   const JagSeg* seg = a.segs + (size_t)k * a.nseg;
   const uint32_t P = seg[a.nseg - 1].cum_pairs;
+  const bool staged = a.nseg <= JG_LDS_SEGS;
+  if (staged) {
+    for (uint32_t i = threadIdx.x; i < a.nseg; i += blockDim.x) cum[i] = seg[i].cum_pairs;
+    __syncthreads();
+  }
   const T* E = a.erow + (((size_t)1 << (a.nrv - k - 1)) - 1);  // E_{nrv-k-1}
   const T* eint = a.eint + (((size_t)1 << a.niv) - 1);         // E_niv of the batch
   // Round k - 1's state: the input for k = 1, else state[(k - 1) & 1].
@@ -193,7 +202,7 @@ __device__ __forceinline__ void jag_row_pairs(const JagArgs<F>& a, uint32_t k, t
     uint32_t lo = 0, hi = a.nseg - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (seg[mid].cum_pairs > p) hi = mid;
+      if ((staged ? cum[mid] : seg[mid].cum_pairs) > p) hi = mid;
       else lo = mid + 1;
     }
     const JagSeg sg = seg[lo];
@@ -235,6 +244,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, 
   using T = typename F::T;
   __shared__ JagLds<F> L;
   __shared__ HopLds warm;
+  __shared__ uint32_t cum[JG_LDS_SEGS];
   auto& sh = L.round;
   const uint32_t tid = threadIdx.x, nblk = gridDim.x - 1;
   JagCtl<F>* jc = a.ctl;
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_row_kernel(JagArgs<F> a, 
   const uint32_t blk = blockIdx.x - 1;
   T acc[JG_NS] = {F::zero(), F::zero(), F::zero()};
   jag_row_pairs<F, FOLD>(a, k, ctl->lam, ctl->r, blk * PASS_THREADS + tid, nblk * PASS_THREADS,
-                         acc);
+                         acc, cum);
   block_reduce<F, JG_NS, PASS_THREADS>(acc, sh.lds, sh.out);
   slot_row_add<F, JG_NS>(a.slots, blk, sh.out);
   __builtin_amdgcn_s_waitcnt(0);
@@ -290,6 +300,7 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a
                                                                     uint32_t kfirst) {
   using T = typename F::T;
   __shared__ JagLds<F> L;
+  __shared__ uint32_t cum[JG_LDS_SEGS];
   auto& sh = L.round;
   const uint32_t tid = threadIdx.x;
   JagCtl<F>* jc = a.ctl;
@@ -304,9 +315,9 @@ __global__ __launch_bounds__(PASS_THREADS) void jagged_batch_kernel(JagArgs<F> a
   for (uint32_t k = kfirst; k < a.nrv; ++k) {
     T acc[JG_NS] = {F::zero(), F::zero(), F::zero()};
     if (k == 0)
-      jag_row_pairs<F, false>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc);
+      jag_row_pairs<F, false>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc, cum);
     else
-      jag_row_pairs<F, true>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc);
+      jag_row_pairs<F, true>(a, k, lam, sh.sm.r, tid, PASS_THREADS, acc, cum);
     block_reduce<F, JG_NS, PASS_THREADS>(acc, sh.lds, sh.out);
     if (tid < 64) {
       T pad = L.pad, eq_adj = L.eq_adj;
