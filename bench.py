@@ -19,8 +19,9 @@ The metric has three parts, all in the one JSON line a default run prints:
   proofs/s, against one proof per call;
 * ``engines``: the sqrt-space, univariate-skip and small-value engines at
   n = 24 against the dense prover on the same tables;
-* ``logup``: a dense LogUp-GKR layer sumcheck (2^22 rows of five tables)
-  and a jagged one (256 segments, row counts ~ U{1..2^14}), BabyBear^4.
+* ``logup``: a dense LogUp-GKR layer sumcheck (2^22 rows of five tables),
+  a whole dense GKR chain over a 2^20-entry first layer, and a jagged layer
+  (256 segments, row counts ~ U{1..2^14}), BabyBear^4.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
                     [--no-batch] [--no-serve] [--no-cpu-baseline] [--no-engines]
@@ -694,12 +695,12 @@ def bench_logup(args, n_dense: int = 22, nseg: int = 256, nrv: int = 14) -> dict
         return FieldArray(rng.integers(0, f.p, size=shape + (4,), dtype=np.uint64)
                           .astype(np.uint32), f)
 
-    def timed(fn):
+    def timed(fn, n=reps):
         fn()
         t = time.perf_counter()
-        for _ in range(reps):
+        for _ in range(n):
             fn()
-        return (time.perf_counter() - t) / reps * 1e3
+        return (time.perf_counter() - t) / n * 1e3
 
     def kernels(fn, kinds):
         profile(ctx, True)
@@ -727,6 +728,16 @@ def bench_logup(args, n_dense: int = 22, nseg: int = 256, nrv: int = 14) -> dict
     out["dense"] = {"rows": N, "tables": 5, "ms_per_layer": ms,
                     "table_elems_per_s": 5 * N / ms * 1e3, **summary(kernels(dense, {"logup", "tail"}))}
     state.free()
+    # a whole dense chain: the pyramid of a 2^20-entry first layer built and
+    # proved on the device, from the host call to the final carry
+    from zorch_amd.logup_gkr import GkrLayer, prove_gkr
+    nb, nr = 4, 16
+    first = GkrLayer(*(rand(1 << (nb + nr)) for _ in range(4)), nb)
+    ch_gkr = ChallengePolicy(f)
+    chain = lambda: prove_gkr(first, Sha256FieldTranscript.new(b"gkr", babybear_mont), ch_gkr)
+    ms = timed(chain, 3)
+    out["gkr_chain"] = {"first_layer_entries": 1 << (nb + nr), "batch_variables": nb,
+                        "layers_proved": nr, "ms_per_chain": ms}
     # jagged layer
     counts = tuple(int(c) for c in rng.integers(1, (1 << nrv) + 1, size=nseg))
     live = sum(counts)

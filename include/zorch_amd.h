@@ -340,6 +340,20 @@ int zs_univariate_skip_round0(zs_ctx* ctx, const zs_prove_args* args, uint32_t s
 int zs_prove_small_value(zs_ctx* ctx, const zs_prove_args* args, uint32_t l0,
                          zs_transcript* transcript, zs_proof* out);
 
+/* ---- dense LogUp-GKR circuit on the device --------------------------------
+ * Replace layer_transition (zorch/logup_gkr/circuit.py:100-140) and
+ * expand_eq_to_hypercube (zorch/poly/eq.py) for device-resident layers. A
+ * layer is a device buffer of five rows of `width` elements each, laid out
+ * as a ZS_MODE_LOGUP stage's tables [eq, n0, d1, n1, d0], so that it is
+ * proved in place (tables_on_device = 1) once row 0 holds its eq table.
+ * zs_logup_transition writes rows 1..4 of `next` (five rows of width / 2)
+ * from rows 1..4 of `layer`; zs_logup_eq writes eq(point, x), x in {0,1}^n,
+ * point[0] on the index MSB, to the 2^n elements at `out` (device); `point`
+ * is n host elements. Both synchronous on the context's stream. */
+int zs_logup_transition(zs_ctx* ctx, uint32_t field, const void* layer, size_t width,
+                        void* next);
+int zs_logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void* out);
+
 /* ---- jagged LogUp-GKR layer --------------------------------------------
  * Replaces prove_jagged_layer / the sumcheck of JaggedGkrLayerRound
  * (zorch/logup_gkr/jagged_prover.py:124-448; round bodies

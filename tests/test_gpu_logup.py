@@ -99,6 +99,29 @@ def test_prove_gkr_matches_oracle(cf, tf, nb, nr):
     assert t_ver.stream_state() == t_dev.stream_state()
 
 
+@pytest.mark.parametrize("cf", ["babybearx4_mont", "goldilocks"])
+def test_device_pyramid_equals_host_circuit(cf):
+    """zs_logup_transition layer by layer against the host circuit, and
+    zs_logup_eq against the host eq expansion."""
+    from zorch_amd.fields import ones
+    from zorch_amd.logup_gkr import build_pyramid
+    from zorch_amd.logup_gkr.prover import device_pyramid, expand_eq_to_hypercube
+    F = OFIELD[cf]
+    rng = random.Random(21)
+    _, dlayer = _layers(F, cf, rng, 2, 7)
+    host = build_pyramid(dlayer)
+    dev = device_pyramid(dlayer, cf)
+    assert len(dev) == len(host)
+    for h, d in zip(host, dev):
+        back = d.to_host()
+        for a, b in ((back.numerator_0, h.numerator_0), (back.numerator_1, h.numerator_1),
+                     (back.denominator_0, h.denominator_0), (back.denominator_1, h.denominator_1)):
+            assert _words(a) == _words(b)
+    pt = _to_fa(F, cf, [F.from_int(rng.randrange(1 << 62)) for _ in range(9)])
+    dev[0].set_eq(pt)
+    assert _words(dev[0].tables.to_host()[0]) == _words(expand_eq_to_hypercube(pt, ones((), pt.field)))
+
+
 def test_multi_block_round_matches_oracle():
     """A layer of 2^13 entries: round 0 (4096 pairs) runs as a multi-block
     launch, the later rounds in one block."""

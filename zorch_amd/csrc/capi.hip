@@ -644,6 +644,26 @@ int zs_prove(zs_ctx* ctx, const zs_prove_args* args, zs_transcript* transcript, 
   return inst->collect(out, transcript);
 }
 
+int zs_logup_transition(zs_ctx* ctx, uint32_t field, const void* layer, size_t width,
+                        void* next) {
+  if (!ctx || !layer || !next) return fail(ZS_EINVAL, "null argument");
+  if (!field_bytes(field)) return fail(ZS_EINVAL, "unknown field");
+  if (width < 2 || (width & (width - 1)))
+    return fail(ZS_EINVAL, "a layer's width must be a power of two of at least 2");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  return logup_transition(ctx, field, layer, width, next);
+}
+
+int zs_logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void* out) {
+  if (!ctx || (n && !point) || !out) return fail(ZS_EINVAL, "null argument");
+  if (!field_bytes(field)) return fail(ZS_EINVAL, "unknown field");
+  if (n > ZS_MAX_ROUNDS) return fail(ZS_EINVAL, "too many coordinates");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  return logup_eq(ctx, field, point, n, out);
+}
+
 int zs_prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args* args, zs_transcript* transcript,
                           zs_jagged_proof* out) {
   if (!ctx || !args || !transcript || !out) return fail(ZS_EINVAL, "null argument");

@@ -386,6 +386,107 @@ class LogupInstance final : public Instance {
   StageArgs<F, F> args_{};
 };
 
+// The fractional-sum circuit on the device. Layers live in 5-row buffers
+// laid out as a LogUp stage's tables [eq, n0, d1, n1, d0] (row stride =
+// the layer's width), so a layer is proved where it was built: row 0 takes
+// the eq table of the layer's point (logup_eq_kernel), rows 1..4 the
+// children. Next layer, pair (2j, 2j + 1) (reference circuit.py:100-140):
+// n0' = n0 d1 + n1 d0 and d0' = d0 d1 at 2j, n1', d1' the same at 2j + 1.
+template <class F>
+__global__ __launch_bounds__(PASS_THREADS) void logup_transition_kernel(
+    const typename F::T* in, size_t in_stride, typename F::T* out, size_t out_stride,
+    size_t half) {
+  using T = typename F::T;
+  for (size_t j = (size_t)blockIdx.x * PASS_THREADS + threadIdx.x; j < half;
+       j += (size_t)gridDim.x * PASS_THREADS) {
+    T n0[2], d1[2], n1[2], d0[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      n0[h] = in[1 * in_stride + 2 * j + h];
+      d1[h] = in[2 * in_stride + 2 * j + h];
+      n1[h] = in[3 * in_stride + 2 * j + h];
+      d0[h] = in[4 * in_stride + 2 * j + h];
+    }
+    out[1 * out_stride + j] = F::add(F::mul(n0[0], d1[0]), F::mul(n1[0], d0[0]));  // n0'
+    out[2 * out_stride + j] = F::mul(d0[1], d1[1]);                                // d1'
+    out[3 * out_stride + j] = F::add(F::mul(n0[1], d1[1]), F::mul(n1[1], d0[1]));  // n1'
+    out[4 * out_stride + j] = F::mul(d0[0], d1[0]);                                // d0'
+  }
+}
+
+// eq(z, x) for x in {0,1}^n, z_0 on the index MSB (reference
+// zorch/poly/eq.py expand_eq_to_hypercube): each entry its product over
+// the coordinates.
+template <class F>
+__global__ __launch_bounds__(PASS_THREADS) void logup_eq_kernel(const typename F::T* z, uint32_t n,
+                                                                typename F::T* out) {
+  using T = typename F::T;
+  __shared__ T zs[64];
+  if (threadIdx.x < n) zs[threadIdx.x] = z[threadIdx.x];
+  __syncthreads();
+  const size_t N = (size_t)1 << n;
+  for (size_t x = (size_t)blockIdx.x * PASS_THREADS + threadIdx.x; x < N;
+       x += (size_t)gridDim.x * PASS_THREADS) {
+    T v = F::one();
+    for (uint32_t i = 0; i < n; ++i)
+      v = F::mul(v, ((x >> (n - 1 - i)) & 1) ? zs[i] : F::sub(F::one(), zs[i]));
+    out[x] = v;
+  }
+}
+
+static uint32_t logup_grid(size_t work) {
+  size_t g = (work + PASS_THREADS - 1) / PASS_THREADS;
+  return (uint32_t)(g < 1 ? 1 : g > 4096 ? 4096 : g);
+}
+
+template <class F>
+static int logup_transition_typed(zs_ctx* ctx, const void* layer, size_t width, void* next) {
+  using T = typename F::T;
+  logup_transition_kernel<F><<<logup_grid(width / 2), PASS_THREADS, 0, ctx->stream>>>(
+      (const T*)layer, width, (T*)next, width / 2, width / 2);
+  ZS_HIP(hipGetLastError());
+  ZS_HIP(hipStreamSynchronize(ctx->stream));
+  return ZS_OK;
+}
+
+template <class F>
+static int logup_eq_typed(zs_ctx* ctx, const void* point, uint32_t n, void* out) {
+  using T = typename F::T;
+  void* zd = nullptr;
+  int rc;
+  if ((rc = ctx->pool->alloc(64 * sizeof(T), &zd))) return rc;
+  std::vector<T> z(n);
+  for (uint32_t i = 0; i < n; ++i) z[i] = load_host_el<F>(point, i);
+  hipError_t e = hipMemcpyAsync(zd, z.data(), n * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    logup_eq_kernel<F><<<logup_grid((size_t)1 << n), PASS_THREADS, 0, ctx->stream>>>(
+        (const T*)zd, n, (T*)out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->pool->release(zd);
+  if (e != hipSuccess) return hip_fail(e, "logup eq table");
+  return ZS_OK;
+}
+
+int logup_transition(zs_ctx* ctx, uint32_t field, const void* layer, size_t width, void* next) {
+  switch (field) {
+    case ZS_F_BABYBEAR: return logup_transition_typed<BB>(ctx, layer, width, next);
+    case ZS_F_BABYBEAR4: return logup_transition_typed<BB4>(ctx, layer, width, next);
+    case ZS_F_GOLDILOCKS: return logup_transition_typed<GL>(ctx, layer, width, next);
+    default: return fail(ZS_EINVAL, "unknown field");
+  }
+}
+
+int logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void* out) {
+  switch (field) {
+    case ZS_F_BABYBEAR: return logup_eq_typed<BB>(ctx, point, n, out);
+    case ZS_F_BABYBEAR4: return logup_eq_typed<BB4>(ctx, point, n, out);
+    case ZS_F_GOLDILOCKS: return logup_eq_typed<GL>(ctx, point, n, out);
+    default: return fail(ZS_EINVAL, "unknown field");
+  }
+}
+
 Instance* make_logup_instance(uint32_t field) {
   switch (field) {
     case ZS_F_BABYBEAR: return new LogupInstance<BB>();

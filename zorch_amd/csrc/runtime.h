@@ -216,6 +216,10 @@ class ServeKind {
 // LogUp stages (logup.hip): tables and challenges in `field`; nullptr for
 // an unsupported field.
 Instance* make_logup_instance(uint32_t field);
+// The dense LogUp-GKR circuit on the device (logup.hip): one transition of
+// a 5-row layer buffer, and the eq table of a point into a row.
+int logup_transition(zs_ctx* ctx, uint32_t field, const void* layer, size_t width, void* next);
+int logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void* out);
 // The jagged LogUp-GKR layer sumcheck (jagged.hip); arguments validated by
 // zs_prove_jagged_layer.
 int prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args& a, zs_transcript* t,

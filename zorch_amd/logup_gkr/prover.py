@@ -10,10 +10,17 @@ call folds, evaluates and hashes one round (``zs_state_round`` of a
 the eq table of the carry's point, the layer's whole sumcheck in one device
 call (``zs_prove``), the final openings observed for the child selector and
 ``fold_carry`` (:159-172). ``bind_output`` is the chain's head (:307-341).
+
+Layers are proved where they live: a ``DeviceLayer`` is one device buffer of
+five rows laid out as the stage's tables [eq, n0, d1, n1, d0];
+``device_pyramid`` uploads the first layer once and builds the others with
+``zs_logup_transition``, and each layer's row 0 receives the eq table of its
+point from ``zs_logup_eq`` before the stage runs on the buffer in place.
 """
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from typing import Any, Sequence
 
@@ -22,7 +29,8 @@ from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field, concatenate, fsum, ones, stack, zeros
 from zorch_amd.logup_gkr.circuit import GkrLayer, LogUpGkrOutput, build_pyramid, extract_outputs
 from zorch_amd.round import prove_rounds
-from zorch_amd.runtime import DeviceFoldState, DeviceTables, log2_strict
+from zorch_amd.runtime import (DeviceFoldState, DeviceTables, default_context, el_buffer,
+                               log2_strict)
 from zorch_amd.sumcheck._device import prove_stage
 
 DEGREE = 3
@@ -164,6 +172,78 @@ def expand_eq_to_hypercube(point: FieldArray, scale: FieldArray) -> FieldArray:
     return cur
 
 
+class DeviceLayer:
+    """A layer in HBM: rows [eq, n0, d1, n1, d0] of `width` challenge-field
+    elements (row 0 is scratch for the eq table of the point it is proved
+    at)."""
+
+    def __init__(self, tables: DeviceTables, num_batch_variables: int) -> None:
+        self.tables = tables
+        self.num_batch_variables = num_batch_variables
+
+    @property
+    def width(self) -> int:
+        return self.tables.shape[1]
+
+    @property
+    def num_variables(self) -> int:
+        return log2_strict(self.width)
+
+    @property
+    def num_row_variables(self) -> int:
+        return self.num_variables - self.num_batch_variables
+
+    @classmethod
+    def upload(cls, layer: GkrLayer, field, ctx=None) -> "DeviceLayer":
+        f = as_field(field)
+        ctx = ctx or default_context()
+        w = layer.numerator_0.shape[0]
+        t = DeviceTables.empty((NUM_FACTORS, w), f, ctx)
+        row_bytes = w * f.itemsize
+        for q, plane in enumerate((layer.numerator_0, layer.denominator_1, layer.numerator_1,
+                                   layer.denominator_0), start=1):
+            words = el_buffer(plane, f)
+            _lib.check(_lib.lib().zs_memcpy_htod(
+                ctx.handle, ctypes.c_void_p(t.data_ptr() + q * row_bytes), words.ctypes.data,
+                row_bytes))
+        return cls(t, layer.num_batch_variables)
+
+    def transition(self) -> "DeviceLayer":
+        if self.num_row_variables < 1:
+            raise ValueError("no row variable left to fold")
+        f = self.tables.field
+        nxt = DeviceTables.empty((NUM_FACTORS, self.width // 2), f, self.tables.ctx)
+        _lib.check(_lib.lib().zs_logup_transition(self.tables.ctx.handle, f.fid,
+                                                  ctypes.c_void_p(self.tables.data_ptr()),
+                                                  self.width, ctypes.c_void_p(nxt.data_ptr())))
+        return DeviceLayer(nxt, self.num_batch_variables)
+
+    def set_eq(self, point: FieldArray) -> None:
+        f = self.tables.field
+        if point.shape[0] != self.num_variables:
+            raise ValueError(f"eq point mismatch: {point.shape[0]} coords for a layer of "
+                             f"{self.num_variables} variables")
+        buf = el_buffer(point, f)
+        _lib.check(_lib.lib().zs_logup_eq(self.tables.ctx.handle, f.fid, buf.ctypes.data,
+                                          point.shape[0], ctypes.c_void_p(self.tables.data_ptr())))
+
+    def to_host(self) -> GkrLayer:
+        rows = self.tables.to_host()
+        return GkrLayer(rows[1], rows[3], rows[4], rows[2], self.num_batch_variables)
+
+    def free(self) -> None:
+        self.tables.free()
+
+
+def device_pyramid(first: GkrLayer, field, ctx=None) -> list[DeviceLayer]:
+    """The circuit on the device: the first layer uploaded once, every
+    transition a launch (zs_logup_transition)."""
+    layers = [DeviceLayer.upload(first, field, ctx)]
+    while layers[-1].num_row_variables > 0:
+        layers.append(layers[-1].transition())
+    return layers
+
+
 def bind_output(output: LogUpGkrOutput, transcript, challenges: ChallengePolicy):
     """Observe the output MLEs and draw the initial claim point in one hop
     (prover.py:307-341)."""
@@ -180,8 +260,8 @@ class GkrLayerRound:
     """Prove one GKR layer (prover.py:344-378); the chain of these, floor
     outward, is the GKR prover."""
 
-    def __init__(self, layer: GkrLayer, challenges: ChallengePolicy) -> None:
-        self.layer = layer
+    def __init__(self, layer, challenges: ChallengePolicy) -> None:
+        self.layer = layer  # a GkrLayer (uploaded when proved) or a DeviceLayer
         self.challenges = challenges
 
     def __call__(self, claim, transcript):
@@ -189,14 +269,21 @@ class GkrLayerRound:
         f = as_field(self.challenges.dtype)
         transcript, lam = self.challenges.sample(transcript)
         lam = lam.normalized()
-        eq = expand_eq_to_hypercube(eval_point, ones((), f))
         lay = self.layer
-        state = stack([eq, lay.numerator_0.astype(f), lay.denominator_1.astype(f),
-                       lay.numerator_1.astype(f), lay.denominator_0.astype(f)])
-        # The stage's claim input feeds only the reduced-claim output, which
-        # this round does not read (the verifier recomputes it).
-        proof = prove_stage(state, zeros((), f), f, _lib.ZS_MODE_LOGUP, DEGREE, transcript,
-                            lam=lam)
+        owned = not isinstance(lay, DeviceLayer)
+        if owned:
+            lay = DeviceLayer.upload(lay, f)
+        elif lay.tables.field is not f:
+            raise ValueError(f"device layer in {lay.tables.field}, challenges in {f}")
+        try:
+            lay.set_eq(eval_point.astype(f))
+            # The stage's claim input feeds only the reduced-claim output, which
+            # this round does not read (the verifier recomputes it).
+            proof = prove_stage(lay.tables, zeros((), f), f, _lib.ZS_MODE_LOGUP, DEGREE,
+                                transcript, lam=lam)
+        finally:
+            if owned:
+                lay.free()
         transcript = proof.transcript
         point = proof.point.normalized()
         _, n0, d1, n1, d0 = (proof.final[i] for i in range(NUM_FACTORS))
@@ -207,10 +294,13 @@ class GkrLayerRound:
 
 def prove_gkr(first: GkrLayer, transcript, challenges: ChallengePolicy):
     """The GKR prover chain over `first`'s pyramid (reference test kit
-    zorch/logup_gkr/testing/__init__.py prove_gkr_with_transcript).
-    Returns (layers, output, layer_proofs, final carry, transcript)."""
-    layers = build_pyramid(first)
-    output = extract_outputs(layers[-1])
+    zorch/logup_gkr/testing/__init__.py prove_gkr_with_transcript), the
+    pyramid built and proved on the device. Returns (layers, output,
+    layer_proofs, final carry, transcript); the layers are DeviceLayers
+    (``to_host()`` reads one back)."""
+    f = as_field(challenges.dtype)
+    layers = device_pyramid(first, f)
+    output = extract_outputs(layers[-1].to_host())
     carry, transcript = bind_output(output, transcript, challenges)
     final, transcript, proofs = prove_rounds(
         [GkrLayerRound(layer, challenges) for layer in reversed(layers[:-1])], carry, transcript)
@@ -219,4 +309,4 @@ def prove_gkr(first: GkrLayer, transcript, challenges: ChallengePolicy):
 
 __all__ = ["logup_combine", "LogupSummand", "RoundMsg", "LogupSumcheckRound", "LayerProof",
            "fold_carry", "eval_mle", "expand_eq_to_hypercube", "bind_output", "GkrLayerRound",
-           "prove_gkr", "DEGREE"]
+           "prove_gkr", "DEGREE", "DeviceLayer", "device_pyramid"]

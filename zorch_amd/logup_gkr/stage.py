@@ -4,9 +4,9 @@ zorch/logup_gkr/stage.py).
 The stage reduces a public output claim (the circuit's output fractions and
 the number of layers the verifier expects) to a claim on the input layer:
 the interleaved numerator and denominator MLEs evaluated at one point. The
-prover builds the pyramid and proves every layer from the floor out (each
-layer's sumcheck one ``ZS_MODE_LOGUP`` device stage); the verifier replays
-the same chain (stage.py:59-128). The layer count is the verifier's own: a
+prover builds the pyramid on the device and proves every layer from the
+floor out (each layer's sumcheck one ``ZS_MODE_LOGUP`` device stage on the
+layer's own buffer); the verifier replays the same chain (stage.py:59-128). The layer count is the verifier's own: a
 witness or proof of another height is rejected before any work.
 """
 
@@ -16,10 +16,10 @@ from dataclasses import dataclass
 from typing import Generic, TypeVar
 
 from zorch_amd.challenge import ChallengePolicy
-from zorch_amd.fields import FieldArray
-from zorch_amd.logup_gkr.circuit import GkrLayer, LogUpGkrOutput, build_pyramid
+from zorch_amd.fields import FieldArray, as_field
+from zorch_amd.logup_gkr.circuit import GkrLayer, LogUpGkrOutput
 from zorch_amd.logup_gkr.prover import GkrLayerRound as ProverLayerRound
-from zorch_amd.logup_gkr.prover import LayerProof, bind_output
+from zorch_amd.logup_gkr.prover import LayerProof, bind_output, device_pyramid
 from zorch_amd.logup_gkr.verifier import GkrLayerRound as VerifierLayerRound
 from zorch_amd.round import prove_rounds, verify_rounds
 from zorch_amd.stage import ProveResult, VerifyResult
@@ -66,7 +66,7 @@ class LogUpGkrProver:
         if witness.num_row_variables != claim.layers:
             raise ValueError(f"claim expects {claim.layers} GKR layers, "
                              f"witness needs {witness.num_row_variables}")
-        pyramid = build_pyramid(witness)
+        pyramid = device_pyramid(witness, as_field(self.challenges.dtype))
         carry, transcript = bind_output(claim.output, transcript, self.challenges)
         carry, transcript, proofs = prove_rounds(
             [ProverLayerRound(layer, self.challenges) for layer in reversed(pyramid[:-1])],

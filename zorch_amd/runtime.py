@@ -120,6 +120,16 @@ class DeviceTables:
         _lib.check(_lib.lib().zs_memcpy_htod(ctx.handle, p, words.ctypes.data, words.nbytes))
         return cls(int(p.value), tables.shape, tables.field, ctx)
 
+    @classmethod
+    def empty(cls, shape: tuple[int, int], field: Any, ctx: Context | None = None) -> "DeviceTables":
+        """Uninitialised device tables of `shape` in `field`."""
+        ctx = ctx or default_context()
+        f = as_field(field)
+        nbytes = int(shape[0]) * int(shape[1]) * f.itemsize
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().zs_device_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        return cls(int(p.value), shape, f, ctx)
+
     def to_host(self) -> FieldArray:
         f = self.field
         out = np.zeros(self.shape + (f.words,), dtype=f.word_dtype)
