@@ -759,6 +759,26 @@ def bench_logup(args, n_dense: int = 22, nseg: int = 256, nrv: int = 14) -> dict
                      "live_elems_per_s": 4 * live / ms * 1e3,
                      **summary(kernels(jag, {"jagged", "jagged_batch"}))}
     resident.free()
+    # a whole jagged chain over that layer: the pyramid built and proved on
+    # the device (row counts halved per layer, over-padded to even)
+    from zorch_amd.logup_gkr import (JaggedGkrLayerRound, bind_output, extract_jagged_outputs,
+                                     jagged_device_pyramid)
+    from zorch_amd.round import prove_rounds
+    sched, c = [], counts
+    while max(c) > 1:
+        c = tuple(x if x == 1 else x + x % 2 for x in ((y + 1) // 2 for y in c))
+        sched.append(c)
+
+    def jchain():
+        layers = jagged_device_pyramid(layer, sched, f)
+        output = extract_jagged_outputs(layers[-1].to_host())
+        carry, t = bind_output(output, Sha256FieldTranscript.new(b"jchain", babybear_mont), ch)
+        return prove_rounds([JaggedGkrLayerRound(lay, ch, caps=caps)
+                             for lay in reversed(layers[:-1])], carry, t)
+
+    ms = timed(jchain, 3)
+    out["jagged_chain"] = {"segments": nseg, "first_layer_live_rows": live,
+                           "layers_proved": len(sched), "ms_per_chain": ms}
     return out
 
 

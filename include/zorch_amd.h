@@ -368,6 +368,18 @@ int zs_logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void
  * the round's eq factor, the virtual rows' mass added in closed form).
  * Planes, challenges and messages share `field`; the transcript is over
  * `field` or its base field. */
+/* One transition of the jagged circuit on the device (replaces
+ * jagged_layer_transition, zorch/logup_gkr/circuit.py:296-470): `planes` are
+ * the four planes n0, n1, d0, d1 of a layer (device, row stride `width`,
+ * segment s holding row_counts[s] rows back to back); `next` receives the
+ * four planes of the next layer (row stride `out_width`) with
+ * out_row_counts[s] >= ceil(row_counts[s] / 2) rows per segment: folded
+ * pairs, then neutral rows, then zeros past the live rows. Counts are host
+ * arrays of num_segments entries. Synchronous on the context's stream. */
+int zs_jagged_transition(zs_ctx* ctx, uint32_t field, uint32_t num_segments,
+                         const uint32_t* row_counts, const uint32_t* out_row_counts,
+                         const void* planes, size_t width, void* next, size_t out_width);
+
 typedef struct zs_jagged_args {
   uint32_t field;              /* ZS_F_* of planes, challenges and messages */
   uint32_t num_segments;       /* 2^niv segments (a power of two) */

@@ -29,7 +29,8 @@ from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field, concatenate, from_storage, fsum, ones, zeros
 from zorch_amd.logup_gkr import (JaggedGkrLayer, JaggedGkrLayerRound, JaggedVerifierLayerRound,
                                  RoundWidthCaps, bind_output, build_jagged_pyramid,
-                                 extract_jagged_outputs, prove_jagged_layer)
+                                 extract_jagged_outputs, jagged_device_pyramid,
+                                 prove_jagged_layer)
 from zorch_amd.logup_gkr.circuit import _interleave
 from zorch_amd.logup_gkr.prover import eval_mle
 from zorch_amd.round import prove_rounds, verify_rounds
@@ -111,8 +112,29 @@ def test_layer_matches_oracle(cf, tf, spec, nrv):
     assert t_dev.stream_state() == tuple(t_or.state())
 
 
+@pytest.mark.parametrize("cf", ["babybearx4_mont", "goldilocks"])
+def test_device_pyramid_equals_host_circuit(cf):
+    """zs_jagged_transition layer by layer against the host circuit, over
+    odd, saturated and over-padded segments."""
+    F = OFIELD[cf]
+    rng = random.Random(12)
+    counts = tuple(rng.randrange(1, 40) for _ in range(16))
+    _, dlayer = _layers(F, cf, rng, counts)
+    sched = OJ.jagged_fold_schedules(counts)
+    host = build_jagged_pyramid(dlayer, sched)
+    dev = jagged_device_pyramid(dlayer, sched, cf)
+    for h, d in zip(host, dev):
+        back = d.to_host()
+        assert back.row_counts == h.row_counts
+        for a, b in ((back.numerator_0, h.numerator_0), (back.numerator_1, h.numerator_1),
+                     (back.denominator_0, h.denominator_0),
+                     (back.denominator_1, h.denominator_1)):
+            assert _words(a) == _words(b)
+
+
 @pytest.mark.parametrize("cf,tf", PAIRS)
-def test_chain_matches_oracle_and_verifies(cf, tf):
+@pytest.mark.parametrize("on_device", [False, True])
+def test_chain_matches_oracle_and_verifies(cf, tf, on_device):
     F, T = OFIELD[cf], OFIELD[tf]
     rng = random.Random(11)
     counts = (3, 1, 5, 2, 6, 1, 8, 4)
@@ -120,8 +142,10 @@ def test_chain_matches_oracle_and_verifies(cf, tf):
     sched = OJ.jagged_fold_schedules(counts)
     ch = ChallengePolicy(cf)
     label = b"jagged-chain"
-    layers = build_jagged_pyramid(dlayer, sched)
-    output = extract_jagged_outputs(layers[-1])
+    layers = (jagged_device_pyramid(dlayer, sched, cf) if on_device
+              else build_jagged_pyramid(dlayer, sched))
+    floor = layers[-1].to_host() if on_device else layers[-1]
+    output = extract_jagged_outputs(floor)
     caps = _caps(counts, len(layers) - 1)
     carry, t = bind_output(output, Sha256FieldTranscript.new(label, tf), ch)
     final, t_dev, proofs = prove_rounds(

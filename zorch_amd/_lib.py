@@ -212,6 +212,7 @@ SIGNATURES = {
                                     ctypes.POINTER(TranscriptC), ctypes.POINTER(ProofC)]),
     "zs_logup_transition": (_INT, [_P, _U32, _P, _SZ, _P]),
     "zs_logup_eq": (_INT, [_P, _U32, _P, _U32, _P]),
+    "zs_jagged_transition": (_INT, [_P, _U32, _U32, _P, _P, _P, _SZ, _P, _SZ]),
     "zs_prove_jagged_layer": (_INT, [_P, ctypes.POINTER(JaggedArgsC),
                                      ctypes.POINTER(TranscriptC), ctypes.POINTER(JaggedProofC)]),
     "zs_state_create": (_INT, [_P, ctypes.POINTER(ProveArgsC), ctypes.POINTER(_P)]),

@@ -664,6 +664,31 @@ int zs_logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void
   return logup_eq(ctx, field, point, n, out);
 }
 
+int zs_jagged_transition(zs_ctx* ctx, uint32_t field, uint32_t num_segments,
+                         const uint32_t* row_counts, const uint32_t* out_row_counts,
+                         const void* planes, size_t width, void* next, size_t out_width) {
+  if (!ctx || !row_counts || !out_row_counts || !planes || !next)
+    return fail(ZS_EINVAL, "null argument");
+  if (!field_bytes(field)) return fail(ZS_EINVAL, "unknown field");
+  if (num_segments == 0) return fail(ZS_EINVAL, "no segments");
+  uint64_t live = 0, out_live = 0;
+  for (uint32_t s = 0; s < num_segments; ++s) {
+    if (out_row_counts[s] < (row_counts[s] + 1) / 2)
+      return fail(ZS_EINVAL, "out row count " + std::to_string(out_row_counts[s]) +
+                                 " truncates a segment of " + std::to_string(row_counts[s]) +
+                                 " rows");
+    live += row_counts[s];
+    out_live += out_row_counts[s];
+  }
+  if (live > width) return fail(ZS_EINVAL, "the row counts exceed the layer's width");
+  if (out_live > out_width || out_width > 0xffffffffull)
+    return fail(ZS_EINVAL, "out_width cannot hold the schedule's live rows");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  return jagged_transition(ctx, field, num_segments, row_counts, out_row_counts, planes, width,
+                           next, out_width);
+}
+
 int zs_prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args* args, zs_transcript* transcript,
                           zs_jagged_proof* out) {
   if (!ctx || !args || !transcript || !out) return fail(ZS_EINVAL, "null argument");

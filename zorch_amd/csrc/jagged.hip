@@ -613,6 +613,104 @@ static int prove_jagged_typed(zs_ctx* ctx, const zs_jagged_args& a, zs_transcrip
   return ZS_OK;
 }
 
+// One transition of the jagged circuit on the device (reference
+// circuit.py:296-470): output slot q of segment s (binary search over the
+// out counts' prefix sums) is row j of the segment; while j < ceil(rc/2) it
+// folds the segment's rows 2j and 2j + 1 (the odd one the neutral fraction
+// past the count), later slots of the out count fold two neutral rows, and
+// slots past the live out rows are zero. Planes n0, n1, d0, d1, row stride
+// = the width.
+struct JagTrSeg {
+  uint32_t cum_out;  // out rows of segments 0..s (inclusive)
+  uint32_t out;      // out rows of segment s
+  uint32_t base;     // first row of segment s in the input
+  uint32_t rows;     // input rows of segment s
+};
+
+template <class F>
+__global__ __launch_bounds__(PASS_THREADS) void jagged_transition_kernel(
+    const JagTrSeg* seg, uint32_t nseg, const typename F::T* in, size_t in_stride,
+    typename F::T* out, size_t out_stride, uint32_t out_width) {
+  using T = typename F::T;
+  const uint32_t live = seg[nseg - 1].cum_out;
+  for (uint32_t q = blockIdx.x * PASS_THREADS + threadIdx.x; q < out_width;
+       q += gridDim.x * PASS_THREADS) {
+    T v[4][2];  // plane, even / odd source
+    if (q >= live) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) out[(size_t)p * out_stride + q] = F::zero();
+      continue;
+    }
+    uint32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (seg[mid].cum_out > q) hi = mid;
+      else lo = mid + 1;
+    }
+    const JagTrSeg sg = seg[lo];
+    const uint32_t j = q - (sg.cum_out - sg.out);
+    const bool folds = j < (sg.rows + 1) / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t r = 2 * j + h;
+      const bool real = folds && r < sg.rows;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        v[p][h] = real ? in[(size_t)p * in_stride + sg.base + r] : jag_neutral<F>(p);
+    }
+    // n0' = n0 d1 + n1 d0 and d0' = d0 d1 at the even row; n1', d1' at the odd.
+    out[0 * out_stride + q] = F::add(F::mul(v[0][0], v[3][0]), F::mul(v[1][0], v[2][0]));
+    out[1 * out_stride + q] = F::add(F::mul(v[0][1], v[3][1]), F::mul(v[1][1], v[2][1]));
+    out[2 * out_stride + q] = F::mul(v[2][0], v[3][0]);
+    out[3 * out_stride + q] = F::mul(v[2][1], v[3][1]);
+  }
+}
+
+template <class F>
+static int jagged_transition_typed(zs_ctx* ctx, uint32_t nseg, const uint32_t* rows,
+                                   const uint32_t* out_rows, const void* planes, size_t width,
+                                   void* next, size_t out_width) {
+  using T = typename F::T;
+  std::vector<JagTrSeg> segs(nseg);
+  uint32_t cum = 0, base = 0;
+  for (uint32_t s = 0; s < nseg; ++s) {
+    cum += out_rows[s];
+    segs[s] = JagTrSeg{cum, out_rows[s], base, rows[s]};
+    base += rows[s];
+  }
+  void* d = nullptr;
+  int rc;
+  if ((rc = ctx->pool->alloc(nseg * sizeof(JagTrSeg), &d))) return rc;
+  hipError_t e = hipMemcpyAsync(d, segs.data(), nseg * sizeof(JagTrSeg), hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess) {
+    size_t g = (out_width + PASS_THREADS - 1) / PASS_THREADS;
+    g = g < 1 ? 1 : g > 4096 ? 4096 : g;
+    jagged_transition_kernel<F><<<(uint32_t)g, PASS_THREADS, 0, ctx->stream>>>(
+        (const JagTrSeg*)d, nseg, (const T*)planes, width, (T*)next, out_width,
+        (uint32_t)out_width);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->pool->release(d);
+  if (e != hipSuccess) return hip_fail(e, "jagged transition");
+  return ZS_OK;
+}
+
+int jagged_transition(zs_ctx* ctx, uint32_t field, uint32_t nseg, const uint32_t* rows,
+                      const uint32_t* out_rows, const void* planes, size_t width, void* next,
+                      size_t out_width) {
+  switch (field) {
+    case ZS_F_BABYBEAR:
+      return jagged_transition_typed<BB>(ctx, nseg, rows, out_rows, planes, width, next, out_width);
+    case ZS_F_BABYBEAR4:
+      return jagged_transition_typed<BB4>(ctx, nseg, rows, out_rows, planes, width, next, out_width);
+    case ZS_F_GOLDILOCKS:
+      return jagged_transition_typed<GL>(ctx, nseg, rows, out_rows, planes, width, next, out_width);
+    default: return fail(ZS_EINVAL, "unknown field");
+  }
+}
+
 int prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args& a, zs_transcript* t,
                        zs_jagged_proof* out) {
   switch (a.field) {

@@ -220,6 +220,10 @@ Instance* make_logup_instance(uint32_t field);
 // a 5-row layer buffer, and the eq table of a point into a row.
 int logup_transition(zs_ctx* ctx, uint32_t field, const void* layer, size_t width, void* next);
 int logup_eq(zs_ctx* ctx, uint32_t field, const void* point, uint32_t n, void* out);
+// One transition of the jagged circuit on the device (jagged.hip).
+int jagged_transition(zs_ctx* ctx, uint32_t field, uint32_t nseg, const uint32_t* rows,
+                      const uint32_t* out_rows, const void* planes, size_t width, void* next,
+                      size_t out_width);
 // The jagged LogUp-GKR layer sumcheck (jagged.hip); arguments validated by
 // zs_prove_jagged_layer.
 int prove_jagged_layer(zs_ctx* ctx, const zs_jagged_args& a, zs_transcript* t,

@@ -8,8 +8,9 @@ from zorch_amd.logup_gkr.circuit import (GkrLayer, JaggedGkrLayer, LogUpGkrOutpu
                                          build_jagged_pyramid, build_pyramid,
                                          extract_jagged_outputs, extract_outputs,
                                          jagged_layer_transition, layer_transition)
-from zorch_amd.logup_gkr.jagged_prover import (JaggedGkrLayerRound, JaggedLayerProof,
-                                               RoundWidthCaps, prove_jagged_layer)
+from zorch_amd.logup_gkr.jagged_prover import (JaggedDeviceLayer, JaggedGkrLayerRound,
+                                               JaggedLayerProof, RoundWidthCaps,
+                                               jagged_device_pyramid, prove_jagged_layer)
 from zorch_amd.logup_gkr.jagged_verifier import JaggedGkrLayerRound as JaggedVerifierLayerRound
 from zorch_amd.logup_gkr.prover import (GkrLayerRound, LayerProof, LogupSumcheckRound,
                                         LogupSummand, bind_output, fold_carry, logup_combine,
@@ -25,4 +26,5 @@ __all__ = ["GkrLayer", "LogUpGkrOutput", "build_pyramid", "extract_outputs", "la
            "GkrProof", "InputLayerClaim", "LogUpGkrProver", "LogUpGkrVerifier",
            "LogUpOutputClaim", "JaggedGkrLayer", "jagged_layer_transition",
            "build_jagged_pyramid", "extract_jagged_outputs", "JaggedGkrLayerRound",
-           "JaggedLayerProof", "RoundWidthCaps", "prove_jagged_layer", "JaggedVerifierLayerRound"]
+           "JaggedLayerProof", "RoundWidthCaps", "prove_jagged_layer", "JaggedVerifierLayerRound",
+           "JaggedDeviceLayer", "jagged_device_pyramid"]

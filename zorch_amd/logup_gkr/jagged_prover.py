@@ -10,7 +10,11 @@ in closed form (jagged_prover.py:1-33). The whole layer sumcheck is one
 device call (``zs_prove_jagged_layer``: one fold-and-sum launch per row
 round with the finisher block, one block for the batch rounds);
 ``JaggedGkrLayerRound`` adds the carry around it as the dense
-``GkrLayerRound`` does (jagged_prover.py:453-548, 671-712).
+``GkrLayerRound`` does (jagged_prover.py:453-548, 671-712). A
+``JaggedDeviceLayer`` keeps a layer's four planes in HBM;
+``jagged_device_pyramid`` uploads the first layer once and runs every
+transition on the device (``zs_jagged_transition``), and the layer rounds
+prove those planes in place.
 """
 
 from __future__ import annotations
@@ -25,7 +29,7 @@ from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field, stack
 from zorch_amd.logup_gkr.circuit import JaggedGkrLayer
 from zorch_amd.logup_gkr.prover import fold_carry
-from zorch_amd.runtime import default_context, el_buffer
+from zorch_amd.runtime import DeviceTables, default_context, el_buffer
 from zorch_amd.sumcheck.jagged.types import RoundWidthCaps
 
 DEGREE = 3
@@ -88,6 +92,69 @@ def _device_prove(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
             type(transcript)(st, transcript.dtype))
 
 
+class JaggedDeviceLayer:
+    """A jagged layer in HBM: the planes n0, n1, d0, d1 (one device buffer,
+    row stride = the live rows) in the challenge field, and the row counts."""
+
+    def __init__(self, planes: DeviceTables, row_counts: tuple) -> None:
+        self.planes = planes
+        self.row_counts = tuple(int(c) for c in row_counts)
+
+    @property
+    def num_batches(self) -> int:
+        return len(self.row_counts)
+
+    @property
+    def num_batch_variables(self) -> int:
+        from zorch_amd.runtime import log2_strict
+        return log2_strict(self.num_batches)
+
+    @property
+    def live(self) -> int:
+        return sum(self.row_counts)
+
+    width = live
+
+    @classmethod
+    def upload(cls, layer: JaggedGkrLayer, field, ctx=None) -> "JaggedDeviceLayer":
+        f = as_field(field)
+        live = layer.live
+        planes = stack([a[:live].astype(f) for a in (layer.numerator_0, layer.numerator_1,
+                                                     layer.denominator_0, layer.denominator_1)])
+        return cls(DeviceTables.from_host(planes.normalized(), ctx), layer.row_counts)
+
+    def transition(self, out_row_counts) -> "JaggedDeviceLayer":
+        out = tuple(int(c) for c in out_row_counts)
+        if len(out) != self.num_batches:
+            raise ValueError(f"schedule must cover all {self.num_batches} batches, got "
+                             f"{len(out)} entries")
+        f = self.planes.field
+        nxt = DeviceTables.empty((4, sum(out)), f, self.planes.ctx)
+        rows = np.ascontiguousarray(np.asarray(self.row_counts, dtype=np.uint32))
+        orows = np.ascontiguousarray(np.asarray(out, dtype=np.uint32))
+        _lib.check(_lib.lib().zs_jagged_transition(
+            self.planes.ctx.handle, f.fid, self.num_batches, rows.ctypes.data, orows.ctypes.data,
+            ctypes.c_void_p(self.planes.data_ptr()), self.live, ctypes.c_void_p(nxt.data_ptr()),
+            sum(out)))
+        return JaggedDeviceLayer(nxt, out)
+
+    def to_host(self) -> JaggedGkrLayer:
+        p = self.planes.to_host()
+        return JaggedGkrLayer(p[0], p[1], p[2], p[3], self.row_counts)
+
+    def free(self) -> None:
+        self.planes.free()
+
+
+def jagged_device_pyramid(first: JaggedGkrLayer, schedules, field, ctx=None):
+    """[first, ..., floor] in HBM: the first layer uploaded once, one
+    zs_jagged_transition per schedule entry (out row counts)."""
+    layers = [JaggedDeviceLayer.upload(first, field, ctx)]
+    for sched in schedules:
+        layers.append(layers[-1].transition(sched))
+    return layers
+
+
 def _check_layer(layer: JaggedGkrLayer, eval_point: FieldArray, caps) -> int:
     if caps is None:
         raise ValueError("a jagged layer proves under caps; pass RoundWidthCaps")
@@ -117,8 +184,13 @@ def prove_jagged_layer(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray
     lam = lam.astype(chal).normalized()
     claim = claim.astype(chal).normalized()
     eval_point = eval_point.astype(chal)
+    resident = None
+    if isinstance(layer, JaggedDeviceLayer):
+        if layer.planes.field is not chal:
+            raise ValueError(f"device layer in {layer.planes.field}, challenges in {chal}")
+        resident = layer.planes
     polys, chals, opens, transcript = _device_prove(layer, lam, claim, eval_point, transcript,
-                                                    chal)
+                                                    chal, resident=resident)
     point = chals[::-1]
     proof = JaggedLayerProof(lam, claim, polys, point, opens[0], opens[1], opens[2], opens[3])
     return point, transcript, proof
