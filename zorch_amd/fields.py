@@ -275,6 +275,13 @@ class FieldArray:
         """The same values with every storage word reduced."""
         return FieldArray(encode(self.field, self.canonical()), self.field)
 
+    def reduced(self) -> "FieldArray":
+        """The same values with every storage word below p: this array when
+        it already is (one comparison pass), else ``normalized()``."""
+        if bool((self.words < np.asarray(self.field.p, dtype=self.words.dtype)).all()):
+            return self
+        return self.normalized()
+
     def tobytes(self) -> bytes:
         return np.ascontiguousarray(self.words).astype(
             np.dtype(self.field.word_dtype).newbyteorder("<")).tobytes()

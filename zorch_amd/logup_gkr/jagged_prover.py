@@ -68,7 +68,7 @@ def _device_prove(layer: JaggedGkrLayer, lam: FieldArray, claim: FieldArray,
     else:
         planes = stack([a[:live].astype(chal) for a in (layer.numerator_0, layer.numerator_1,
                                                         layer.denominator_0, layer.denominator_1)])
-        words = np.ascontiguousarray(planes.normalized().words)
+        words = np.ascontiguousarray(planes.reduced().words)
         on_dev, ptr = 0, words.ctypes.data
     counts = np.ascontiguousarray(np.asarray(layer.row_counts, dtype=np.uint32))
     pt = el_buffer(eval_point, chal)
@@ -118,10 +118,17 @@ class JaggedDeviceLayer:
     @classmethod
     def upload(cls, layer: JaggedGkrLayer, field, ctx=None) -> "JaggedDeviceLayer":
         f = as_field(field)
+        ctx = ctx or default_context()
         live = layer.live
-        planes = stack([a[:live].astype(f) for a in (layer.numerator_0, layer.numerator_1,
-                                                     layer.denominator_0, layer.denominator_1)])
-        return cls(DeviceTables.from_host(planes.normalized(), ctx), layer.row_counts)
+        t = DeviceTables.empty((4, live), f, ctx)
+        row_bytes = live * f.itemsize
+        for q, plane in enumerate((layer.numerator_0, layer.numerator_1, layer.denominator_0,
+                                   layer.denominator_1)):
+            words = np.ascontiguousarray(plane[:live].astype(f).reduced().words)
+            _lib.check(_lib.lib().zs_memcpy_htod(
+                ctx.handle, ctypes.c_void_p(t.data_ptr() + q * row_bytes), words.ctypes.data,
+                row_bytes))
+        return cls(t, layer.row_counts)
 
     def transition(self, out_row_counts) -> "JaggedDeviceLayer":
         out = tuple(int(c) for c in out_row_counts)

@@ -24,6 +24,8 @@ import ctypes
 from dataclasses import dataclass
 from typing import Any, Sequence
 
+import numpy as np
+
 from zorch_amd import _lib
 from zorch_amd.challenge import ChallengePolicy
 from zorch_amd.fields import FieldArray, as_field, concatenate, fsum, ones, stack, zeros
@@ -202,7 +204,7 @@ class DeviceLayer:
         row_bytes = w * f.itemsize
         for q, plane in enumerate((layer.numerator_0, layer.denominator_1, layer.numerator_1,
                                    layer.denominator_0), start=1):
-            words = el_buffer(plane, f)
+            words = np.ascontiguousarray(plane.astype(f).reduced().words)
             _lib.check(_lib.lib().zs_memcpy_htod(
                 ctx.handle, ctypes.c_void_p(t.data_ptr() + q * row_bytes), words.ctypes.data,
                 row_bytes))
