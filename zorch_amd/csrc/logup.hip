@@ -70,6 +70,48 @@ __device__ __forceinline__ void logup_pairs(const StageArgs<F, F>& a, const type
   }
 }
 
+// A small round in one block (4 x pairs <= block threads): thread 4j + t
+// folds pair j (only t = 0 writes the folded entries) and adds the summand
+// at node t alone, so a pair's four combines run side by side.
+template <class F, bool FOLD>
+__device__ __forceinline__ void logup_pairs_split(const StageArgs<F, F>& a,
+                                                  const typename F::T* src, size_t src_stride,
+                                                  uint32_t L, typename F::T lam, typename F::T r,
+                                                  typename F::T (&acc)[LG_NA]) {
+  using T = typename F::T;
+  const size_t half = L >> 1;
+  const size_t j = threadIdx.x >> 2;
+  const uint32_t t = threadIdx.x & 3u;
+  if (j >= half) return;
+  T v[LG_TABLES];
+#pragma unroll
+  for (int q = 0; q < LG_TABLES; ++q) {
+    const T* s = src + (size_t)q * src_stride;
+    T x0, x1;
+    if constexpr (FOLD) {
+      const T a0 = s[j], a1 = s[j + L], b0 = s[j + half], b1 = s[j + half + L];
+      x0 = F::add(a0, F::mul(r, F::sub(a1, a0)));
+      x1 = F::add(b0, F::mul(r, F::sub(b1, b0)));
+      if (t == 0) {
+        T* d = a.work + (size_t)q * a.work_stride;
+        d[j] = x0;
+        d[j + half] = x1;
+      }
+    } else {
+      x0 = s[j];
+      x1 = s[j + half];
+    }
+    const T df = F::sub(x1, x0);
+    T x = x0;
+    for (uint32_t i = 0; i < t; ++i) x = F::add(x, df);
+    v[q] = x;
+  }
+  const T c = logup_combine<F>(lam, v);
+#pragma unroll
+  for (int i = 0; i < LG_NA; ++i)
+    if ((uint32_t)i == t) acc[i] = F::add(acc[i], c);
+}
+
 // Round `round` of a LogUp stage as 1 + nblk blocks. Blocks 1..nblk stream
 // the pairs (logup_pairs), add their sums to the slot rows and arrive; block
 // 0 warms the hop on a scratch copy of the transcript state (the finisher's
@@ -148,12 +190,17 @@ __global__ __launch_bounds__(PASS_THREADS) void logup_tail_kernel(StageArgs<F, F
 #pragma unroll
     for (int t = 0; t < LG_NA; ++t) acc[t] = F::zero();
     const T r = sh.sm.r;
-    if (round == 0)
-      logup_pairs<F, false>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
-    else if (round == 1)
-      logup_pairs<F, true>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
-    else
-      logup_pairs<F, true>(a, a.work, a.work_stride, L, lam, r, tid, PASS_THREADS, acc);
+    const bool split = (size_t)(L >> 1) * 4 <= PASS_THREADS;
+    if (round == 0) {
+      if (split) logup_pairs_split<F, false>(a, a.in, a.in_stride, L, lam, r, acc);
+      else logup_pairs<F, false>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
+    } else if (round == 1) {
+      if (split) logup_pairs_split<F, true>(a, a.in, a.in_stride, L, lam, r, acc);
+      else logup_pairs<F, true>(a, a.in, a.in_stride, L, lam, r, tid, PASS_THREADS, acc);
+    } else {
+      if (split) logup_pairs_split<F, true>(a, a.work, a.work_stride, L, lam, r, acc);
+      else logup_pairs<F, true>(a, a.work, a.work_stride, L, lam, r, tid, PASS_THREADS, acc);
+    }
     // The folded entries reach the next round's readers through the barriers
     // (one workgroup).
     block_reduce<F, LG_NA, PASS_THREADS>(acc, sh.lds, sh.out);
