@@ -150,6 +150,15 @@ def c_sub(field: Field, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         return np.where(a >= b, a - b, a + (p - b))
 
 
+_BB4_SMALL = 1024
+# _BB4_FOLD[4 i + j, k]: the weight of a_i b_j in limb k of a product in
+# BB[X] / (X^4 - 11).
+_BB4_FOLD = np.zeros((16, 4), dtype=np.uint64)
+for _i in range(4):
+    for _j in range(4):
+        _BB4_FOLD[4 * _i + _j, (_i + _j) % 4] = 11 if _i + _j >= 4 else 1
+
+
 def c_mul(field: Field, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """Element-wise product of canonical arrays with trailing limb axes."""
     p = np.uint64(BB_P)
@@ -158,8 +167,22 @@ def c_mul(field: Field, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     if field is babybear_mont:
         return (a * b) % p
     # X^4 = 11 quartic extension, schoolbook on canonical limbs.
+    if a.shape == (4,) and b.shape == (4,):
+        x, y, P = a.tolist(), b.tolist(), BB_P
+        out = [0, 0, 0, 0]
+        for i in range(4):
+            for j in range(4):
+                out[(i + j) & 3] += x[i] * y[j] * (11 if i + j >= 4 else 1)
+        return np.array([v % P for v in out], dtype=np.uint64)
     a = np.broadcast_to(a, np.broadcast_shapes(a.shape, b.shape))
     b = np.broadcast_to(b, a.shape)
+    if a.size <= _BB4_SMALL:
+        # Few elements (challenges, claims, openings): the sixteen limb
+        # products at once, each reduced below p, then collected into the
+        # four output limbs by the weights of X^(i+j) mod X^4 - 11 (sums stay
+        # below 2^38).
+        prod = (a[..., :, None] * b[..., None, :]) % p
+        return (prod.reshape(a.shape[:-1] + (16,)) @ _BB4_FOLD) % p
     out = np.zeros(a.shape, dtype=np.uint64)
     for k in range(4):
         acc = np.zeros(a.shape[:-1], dtype=np.uint64)
