@@ -17,7 +17,7 @@ namespace zs {
 constexpr int LG_TABLES = 5;  // [eq, n0, d1, n1, d0]
 constexpr int LG_NA = 4;      // message nodes t = 0..3 (degree 3)
 // Rounds with at most this many pairs run in one block (logup_tail_kernel).
-constexpr size_t LG_TAIL_PAIRS = 2048;
+constexpr size_t LG_TAIL_PAIRS = 512;
 
 // eq (lam (n0 d1 + n1 d0) + d0 d1) on v = [eq, n0, d1, n1, d0].
 template <class F>
